@@ -1,0 +1,186 @@
+/*
+ * raja_cdna4.h -- C ABI of libraja_cdna4.so, the MI355X (gfx950) execution
+ * library behind the RAJA::cdna4_* policy family.
+ *
+ * RAJA has no C ABI for its algorithms: user loop bodies are compiled into
+ * kernels by the C++ templates in include/RAJA/ (forall, reducers).  The
+ * fixed-type primitives behind scan, sort, span reductions, segmented
+ * (batched) work and request serving are exported here with plain pointers
+ * and sizes, and the C++ front-end calls them for pointer + arithmetic +
+ * less/greater cases -- the same envelope the reference's HIP backend accepts
+ * (include/RAJA/policy/hip/sort.hpp:356-401).
+ *
+ * Conventions for every entry point:
+ *   - returns RAJA_CDNA4_SUCCESS (0) or an error code; raja_cdna4_last_error()
+ *     gives the message of the calling thread's last failure;
+ *   - data pointers are caller-owned device memory (or any memory the GPU
+ *     can address); temporaries come from the library's per-stream scratch;
+ *   - work is enqueued asynchronously on `stream` (a hipStream_t, NULL for
+ *     the null stream) unless the entry point says otherwise;
+ *   - calls on different streams may run concurrently.
+ */
+#ifndef RAJA_CDNA4_H
+#define RAJA_CDNA4_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status, types, operators ------------------------------------------- */
+
+enum raja_cdna4_status
+{
+  RAJA_CDNA4_SUCCESS = 0,
+  RAJA_CDNA4_ERROR_INVALID_ARGUMENT = 1,
+  RAJA_CDNA4_ERROR_UNSUPPORTED = 2,
+  RAJA_CDNA4_ERROR_HIP = 3,
+  RAJA_CDNA4_ERROR_OUT_OF_MEMORY = 4,
+  RAJA_CDNA4_ERROR_ADMISSION = 5
+};
+
+/* Element types.  Values of sort_pairs are opaque payloads of val_bytes. */
+enum raja_cdna4_dtype
+{
+  RAJA_CDNA4_I32 = 0,
+  RAJA_CDNA4_I64 = 1,
+  RAJA_CDNA4_U32 = 2,
+  RAJA_CDNA4_U64 = 3,
+  RAJA_CDNA4_F32 = 4,
+  RAJA_CDNA4_F64 = 5
+};
+
+/* Binary operators of RAJA::operators (include/RAJA/util/Operators.hpp):
+ * plus (:362-377), minimum (:555-574), maximum (:576-595),
+ * bit_or/bit_and (:499-527; integer types only). */
+enum raja_cdna4_op
+{
+  RAJA_CDNA4_OP_PLUS = 0,
+  RAJA_CDNA4_OP_MIN = 1,
+  RAJA_CDNA4_OP_MAX = 2,
+  RAJA_CDNA4_OP_BIT_OR = 3,
+  RAJA_CDNA4_OP_BIT_AND = 4
+};
+
+/* ---- runtime -------------------------------------------------------------- */
+
+/* Select `device` for the calling thread and cache its properties.
+ * Replaces camp::resources::Hip device selection and the hipInfo set-up of
+ * include/RAJA/policy/hip/MemUtils_HIP.hpp:166-174. */
+int raja_cdna4_init(int device);
+
+/* Text of the calling thread's last error (never NULL). */
+const char* raja_cdna4_last_error(void);
+
+/* Library version string. */
+const char* raja_cdna4_version(void);
+
+/* Compute units of the current device (256 on MI355X). */
+int raja_cdna4_num_cus(void);
+
+/* Wait for all work on `stream`.  Replaces resources::Hip::wait() /
+ * synchronize_impl(hip_synchronize) (include/RAJA/policy/hip/synchronize.hpp:40-43). */
+int raja_cdna4_stream_synchronize(void* stream);
+
+/* Release the per-stream scratch the library holds for `stream` (call before
+ * destroying a stream that was passed to the library). */
+int raja_cdna4_stream_release(void* stream);
+
+/* Device and pinned-host allocation through the library's pools; used by the
+ * C++ front-end's resources::Cdna4 and reducer storage.  Replaces the
+ * allocators and mempools of include/RAJA/policy/hip/MemUtils_HIP.hpp:69-160
+ * and include/RAJA/util/basic_mempool.hpp:54-454. */
+int raja_cdna4_device_malloc(void** ptr, size_t bytes);
+int raja_cdna4_device_free(void* ptr);
+int raja_cdna4_pinned_malloc(void** ptr, size_t bytes);
+int raja_cdna4_pinned_free(void* ptr);
+
+/* Kernel timing: when enabled, every library kernel launch is bracketed by
+ * HIP events on its launch stream; times accumulate per kernel name
+ * ("onesweep_pass", "radix_hist", "exclusive_scan", "reduce_sum", ...).
+ * Counterpart of the reference's per-launch plugin hooks
+ * (include/RAJA/util/PluginStrategy.hpp:20-40) for timing tools. */
+int raja_cdna4_timing_enable(int on);
+int raja_cdna4_timing_reset(void);
+int raja_cdna4_timing_query(const char* kernel, int64_t* count, double* total_ms,
+                            double* min_ms, double* max_ms);
+/* Comma-separated names of kernels with timing records. */
+int raja_cdna4_timing_names(char* buf, size_t len);
+
+/* ---- synthetic data (portable counter-based generator) --------------------- */
+/* Element i takes splitmix64((seed << 40) + i) (published splitmix64 output
+ * function).  kind 0: u32 top `bits` bits (a = bits); kind 1: i32 uniform in
+ * [a, b); kind 2: i64 uniform in [a, b); kind 3: f64 U[a, b); kind 4: f64
+ * N(0,1) by Box-Muller on counters 2k, 2k+1; kind 5: i32 iota (a + i);
+ * kind 6: f64 a + i; kind 7: u32 constant a. */
+int raja_cdna4_fill(int kind, void* out, int64_t n, uint64_t seed, double a,
+                    double b, void* stream);
+
+/* ---- reductions over a contiguous span ------------------------------------ */
+
+/* out_dev[0] = op(*init, fold_op(x[0..n))) with RAJA's identity for the
+ * empty fold; min/max keep the init value on ties.  Deterministic: the
+ * combine tree depends only on n.  Replaces ReduceSum/Min/Max/BitOr/BitAnd
+ * over a forall (include/RAJA/policy/hip/reduce.hpp:960-1161,
+ * include/RAJA/policy/hip/params/reduce.hpp:19-60) for span inputs.
+ * `init` is a host pointer to one element; `out_dev` a device pointer. */
+int raja_cdna4_reduce(int dtype, int op, const void* x, int64_t n,
+                      const void* init, void* out_dev, void* stream);
+
+/* MinLoc / MaxLoc (is_max = 0 / 1): value and index of the first extremum in
+ * index order, combined with (init_val, init_loc), which wins ties
+ * (include/RAJA/pattern/detail/reduce.hpp:127-358; ValLoc semantics of
+ * include/RAJA/pattern/params/params_base.hpp:9-53).  out_val_dev holds one
+ * element, out_loc_dev one int64. */
+int raja_cdna4_reduce_loc(int dtype, int is_max, const void* x, int64_t n,
+                          const void* init_val, int64_t init_loc,
+                          void* out_val_dev, int64_t* out_loc_dev, void* stream);
+
+/* One read of x feeding ReduceSum, ReduceMin, ReduceMax and ReduceMinLoc at
+ * once (the four reducers of one forall; identities as initial values). */
+typedef struct raja_cdna4_multi_f64
+{
+  double sum;
+  double min;
+  double max;
+  double minloc_val;
+  int64_t minloc_loc;
+  int64_t pad;
+} raja_cdna4_multi_f64;
+
+int raja_cdna4_multi_reduce_f64(const double* x, int64_t n,
+                                raja_cdna4_multi_f64* out_dev, void* stream);
+
+/* ---- scans ----------------------------------------------------------------- */
+/* RAJA::impl::scan::inclusive / inclusive_inplace
+ * (include/RAJA/policy/hip/scan.hpp:51-160): out[0] = in[0],
+ * out[i] = op(out[i-1], in[i]).  in == out is the in-place variant.
+ * op is PLUS, MIN or MAX.  Integer results are bit-exact. */
+int raja_cdna4_inclusive_scan(int dtype, int op, const void* in, void* out,
+                              int64_t n, void* stream);
+
+/* RAJA::impl::scan::exclusive / exclusive_inplace
+ * (include/RAJA/policy/hip/scan.hpp:162-274): out[0] = *init,
+ * out[i] = op(out[i-1], in[i-1]).  `init` is a host pointer to one element. */
+int raja_cdna4_exclusive_scan(int dtype, int op, const void* in, void* out,
+                              int64_t n, const void* init, void* stream);
+
+/* ---- sorts ------------------------------------------------------------------- */
+/* RAJA::impl::sort::{stable,unstable}_pairs (include/RAJA/policy/hip/sort.hpp:
+ * 417-495, 653-663): sort keys ascending (descending = 0, operators::less) or
+ * descending (1, operators::greater); values follow their keys.  The LSD
+ * radix sort is stable, so both variants give the stable result.  vals may be
+ * NULL (keys only: RAJA::impl::sort::{stable,unstable}, sort.hpp:77-356).
+ * val_bytes is 4 or 8.  Signed and floating keys order as operator< does
+ * (-0.0 sorts before +0.0).  Keys with every digit pass constant are not
+ * re-sorted for that pass. */
+int raja_cdna4_sort_pairs(int key_dtype, void* keys, void* vals, int val_bytes,
+                          int64_t n, int descending, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAJA_CDNA4_H */

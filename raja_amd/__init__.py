@@ -1,0 +1,300 @@
+"""raja_amd -- Python bindings of libraja_cdna4.so, the MI355X execution library
+behind the RAJA::cdna4_* policy family.
+
+The names mirror RAJA's pattern interface for the hot path
+(include/RAJA/pattern/scan.hpp, include/RAJA/pattern/sort.hpp,
+include/RAJA/pattern/reduce.hpp): ``inclusive_scan(in, out, op)``,
+``exclusive_scan(in, out, op, init)``, the ``*_inplace`` variants,
+``sort/stable_sort/sort_pairs/stable_sort_pairs(keys[, vals], comparator)``
+and span reductions.  Arguments are torch tensors on a GPU (torch is used for
+device memory and streams only); every call goes through the C ABI declared
+in include/raja_cdna4.h and runs a hand-written gfx950 kernel.  There is no
+fallback: if the library is missing or the tensor is not on a GPU, the call
+raises.
+"""
+import ctypes
+import os
+
+import torch  # loads the HIP runtime the library then binds to (same soname)
+
+__all__ = [
+    "operators", "lib", "RajaError", "inclusive_scan", "exclusive_scan",
+    "inclusive_scan_inplace", "exclusive_scan_inplace", "sort", "stable_sort",
+    "sort_pairs", "stable_sort_pairs", "reduce", "reduce_sum", "reduce_min",
+    "reduce_max", "reduce_minloc", "reduce_maxloc", "multi_reduce_f64", "fill",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libraja_cdna4.so")
+
+# enum raja_cdna4_dtype / raja_cdna4_op
+_DTYPES = {
+    torch.int32: 0, torch.int64: 1, torch.uint32: 2, torch.uint64: 3,
+    torch.float32: 4, torch.float64: 5,
+}
+
+
+class operators:
+    """RAJA::operators tags (include/RAJA/util/Operators.hpp)."""
+    plus = 0
+    minimum = 1
+    maximum = 2
+    bit_or = 3
+    bit_and = 4
+    less = "less"
+    greater = "greater"
+
+
+class RajaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def _declare(L):
+    vp, i64, i32, u64, dbl = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                              ctypes.c_uint64, ctypes.c_double)
+    sigs = {
+        "raja_cdna4_init": [i32],
+        "raja_cdna4_num_cus": [],
+        "raja_cdna4_stream_synchronize": [vp],
+        "raja_cdna4_stream_release": [vp],
+        "raja_cdna4_fill": [i32, vp, i64, u64, dbl, dbl, vp],
+        "raja_cdna4_reduce": [i32, i32, vp, i64, vp, vp, vp],
+        "raja_cdna4_reduce_loc": [i32, i32, vp, i64, vp, i64, vp, vp, vp],
+        "raja_cdna4_multi_reduce_f64": [vp, i64, vp, vp],
+        "raja_cdna4_inclusive_scan": [i32, i32, vp, vp, i64, vp],
+        "raja_cdna4_exclusive_scan": [i32, i32, vp, vp, i64, vp, vp],
+        "raja_cdna4_sort_pairs": [i32, vp, vp, i32, i64, i32, vp],
+        "raja_cdna4_timing_enable": [i32],
+        "raja_cdna4_timing_reset": [],
+        "raja_cdna4_timing_query": [ctypes.c_char_p, ctypes.POINTER(i64),
+                                    ctypes.POINTER(dbl), ctypes.POINTER(dbl),
+                                    ctypes.POINTER(dbl)],
+        "raja_cdna4_timing_names": [ctypes.c_char_p, ctypes.c_size_t],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.raja_cdna4_last_error.restype = ctypes.c_char_p
+    L.raja_cdna4_last_error.argtypes = []
+    L.raja_cdna4_version.restype = ctypes.c_char_p
+    L.raja_cdna4_version.argtypes = []
+
+
+def lib():
+    """Load libraja_cdna4.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RajaError("libraja_cdna4.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise RajaError("raja_cdna4 error %d: %s" % (
+            status, lib().raja_cdna4_last_error().decode()))
+
+
+_initialized = set()
+
+
+def _prep(*tensors):
+    """Validate GPU tensors and return the current stream handle."""
+    dev = None
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise RajaError("raja_amd operates on GPU tensors only")
+        if not t.is_contiguous():
+            raise RajaError("spans must be contiguous")
+        dev = t.device.index if dev is None else dev
+    if dev is None:
+        dev = torch.cuda.current_device()
+    if dev not in _initialized:
+        _check(lib().raja_cdna4_init(dev))
+        _initialized.add(dev)
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _dt(t):
+    try:
+        return _DTYPES[t.dtype]
+    except KeyError:
+        raise RajaError("unsupported dtype %s" % t.dtype)
+
+
+def _host_scalar(value, dtype):
+    buf = torch.tensor([value], dtype=dtype)
+    return buf, ctypes.c_void_p(buf.data_ptr())
+
+
+_IDENT = {
+    operators.plus: lambda dt: 0,
+    operators.minimum: lambda dt: (torch.finfo(dt).max if dt.is_floating_point
+                                   else torch.iinfo(dt).max),
+    operators.maximum: lambda dt: (-torch.finfo(dt).max if dt.is_floating_point
+                                   else torch.iinfo(dt).min),
+}
+
+
+# ---- scans (include/RAJA/pattern/scan.hpp:49-441) ---------------------------
+
+def inclusive_scan(in_, out, op=operators.plus):
+    if in_.numel() != out.numel() or in_.dtype != out.dtype:
+        raise RajaError("in and out spans must match in length and type")
+    s = _prep(in_, out)
+    _check(lib().raja_cdna4_inclusive_scan(_dt(in_), op, in_.data_ptr(), out.data_ptr(),
+                                           in_.numel(), s))
+    return out
+
+
+def exclusive_scan(in_, out, op=operators.plus, init=None):
+    if in_.numel() != out.numel() or in_.dtype != out.dtype:
+        raise RajaError("in and out spans must match in length and type")
+    s = _prep(in_, out)
+    if init is None:
+        init = _IDENT[op](in_.dtype)
+    hold, p = _host_scalar(init, in_.dtype)
+    _check(lib().raja_cdna4_exclusive_scan(_dt(in_), op, in_.data_ptr(), out.data_ptr(),
+                                           in_.numel(), p, s))
+    return out
+
+
+def inclusive_scan_inplace(x, op=operators.plus):
+    return inclusive_scan(x, x, op)
+
+
+def exclusive_scan_inplace(x, op=operators.plus, init=None):
+    return exclusive_scan(x, x, op, init)
+
+
+# ---- sorts (include/RAJA/pattern/sort.hpp:49-453) ------------------------------
+
+def sort_pairs(keys, vals=None, comparator=operators.less):
+    if comparator not in (operators.less, operators.greater):
+        raise RajaError("GPU sorts accept operators.less or operators.greater")
+    if vals is not None and vals.numel() != keys.numel():
+        raise RajaError("keys and values must have the same length")
+    s = _prep(keys, vals)
+    vb = 0 if vals is None else vals.element_size()
+    _check(lib().raja_cdna4_sort_pairs(_dt(keys), keys.data_ptr(),
+                                       None if vals is None else vals.data_ptr(), vb,
+                                       keys.numel(), 1 if comparator == operators.greater else 0,
+                                       s))
+    return keys, vals
+
+
+stable_sort_pairs = sort_pairs  # the LSD radix sort is stable
+
+
+def sort(keys, comparator=operators.less):
+    sort_pairs(keys, None, comparator)
+    return keys
+
+
+stable_sort = sort
+
+
+# ---- reductions (include/RAJA/pattern/reduce.hpp:79-233) ----------------------
+
+def reduce(x, op, init=None, out=None):
+    """Device result op(init, fold_op(x)) as a 1-element tensor."""
+    s = _prep(x)
+    if init is None:
+        init = {operators.bit_or: 0, operators.bit_and: -1}.get(op, None)
+        if init is None:
+            init = _IDENT[op](x.dtype)
+    hold, p = _host_scalar(init, x.dtype)
+    if out is None:
+        out = torch.empty(1, dtype=x.dtype, device=x.device)
+    _check(lib().raja_cdna4_reduce(_dt(x), op, x.data_ptr(), x.numel(), p, out.data_ptr(), s))
+    return out
+
+
+def reduce_sum(x, init=0):
+    return reduce(x, operators.plus, init)
+
+
+def reduce_min(x, init=None):
+    return reduce(x, operators.minimum, init)
+
+
+def reduce_max(x, init=None):
+    return reduce(x, operators.maximum, init)
+
+
+def _reduce_loc(x, is_max, init_val, init_loc):
+    s = _prep(x)
+    if init_val is None:
+        init_val = _IDENT[operators.maximum if is_max else operators.minimum](x.dtype)
+    hold, p = _host_scalar(init_val, x.dtype)
+    val = torch.empty(1, dtype=x.dtype, device=x.device)
+    loc = torch.empty(1, dtype=torch.int64, device=x.device)
+    _check(lib().raja_cdna4_reduce_loc(_dt(x), is_max, x.data_ptr(), x.numel(), p, init_loc,
+                                       val.data_ptr(), loc.data_ptr(), s))
+    return val, loc
+
+
+def reduce_minloc(x, init_val=None, init_loc=-1):
+    return _reduce_loc(x, 0, init_val, init_loc)
+
+
+def reduce_maxloc(x, init_val=None, init_loc=-1):
+    return _reduce_loc(x, 1, init_val, init_loc)
+
+
+def multi_reduce_f64(x, out=None):
+    """(sum, min, max, minloc_val, minloc_loc) of FP64 x in one pass; returns a
+    6-slot float64 device tensor (slot 4 holds the int64 loc bits)."""
+    if x.dtype != torch.float64:
+        raise RajaError("multi_reduce_f64 needs float64")
+    s = _prep(x)
+    if out is None:
+        out = torch.empty(6, dtype=torch.float64, device=x.device)
+    _check(lib().raja_cdna4_multi_reduce_f64(x.data_ptr(), x.numel(), out.data_ptr(), s))
+    return out
+
+
+def unpack_multi(out):
+    h = out.cpu()
+    loc = h[4:5].view(torch.int64).item()
+    return {"sum": h[0].item(), "min": h[1].item(), "max": h[2].item(),
+            "minloc_val": h[3].item(), "minloc_loc": loc}
+
+
+# ---- synthetic inputs -------------------------------------------------------------
+
+FILL_U32_BITS, FILL_I32, FILL_I64, FILL_F64_UNIFORM, FILL_F64_NORMAL, FILL_I32_IOTA, \
+    FILL_F64_IOTA, FILL_U32_CONST = range(8)
+
+
+def fill(kind, out, seed=0, a=0.0, b=0.0):
+    s = _prep(out)
+    _check(lib().raja_cdna4_fill(kind, out.data_ptr(), out.numel(), seed, float(a), float(b), s))
+    return out
+
+
+# ---- kernel timing (HIP events on the launch stream) ----------------------------------
+
+def timing_enable(on=True):
+    _check(lib().raja_cdna4_timing_enable(1 if on else 0))
+
+
+def timing_reset():
+    _check(lib().raja_cdna4_timing_reset())
+
+
+def timing_query(name):
+    c = ctypes.c_int64()
+    t, mn, mx = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _check(lib().raja_cdna4_timing_query(name.encode(), ctypes.byref(c), ctypes.byref(t),
+                                         ctypes.byref(mn), ctypes.byref(mx)))
+    return {"count": c.value, "total_ms": t.value, "min_ms": mn.value, "max_ms": mx.value}

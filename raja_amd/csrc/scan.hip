@@ -1,0 +1,353 @@
+// Single-pass decoupled-lookback scans (inclusive / exclusive, plus / min /
+// max; the in-place variants are in == out).
+//
+// Reference: RAJA's HIP backend delegates to rocprim::inclusive_scan /
+// exclusive_scan (include/RAJA/policy/hip/scan.hpp:51-274); the semantics
+// are those of include/RAJA/policy/sequential/scan.hpp:43-153.  This kernel
+// reads every element once and writes it once (8 B/elem for int32):
+//   * a tile is 4 waves x 64 lanes x V vectors of 16 bytes; wave w owns a
+//     contiguous quarter, lane l loads vector k at ((k*64 + l) * VEC), so
+//     every load and store is a fully coalesced dwordx4;
+//   * each vector is scanned in registers, vector totals are scanned across
+//     the wave with DPP (one wave scan per k), waves are combined in LDS;
+//   * the tile aggregate is published at once, then wave 0 looks back over
+//     up to 64 predecessor tiles per step (one lane per tile) until it meets
+//     an inclusive prefix;
+//   * descriptors carry an epoch bumped by the previous scan kernel's last
+//     block, so they never need clearing (also under graph replay).
+#include "ops.hpp"
+#include "runtime.hpp"
+
+namespace
+{
+using namespace RAJA::cdna4::dev;
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / wave_size;
+constexpr uint32_t kAgg = 1u, kInc = 2u;
+
+template<typename T>
+struct ScanShape
+{
+  static constexpr int VEC = 16 / sizeof(T);
+  static constexpr int V = (sizeof(T) == 4) ? 4 : 8;  // vectors per lane
+  static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
+  static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
+  typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+
+// Descriptor of one tile: one 64-bit word per 32-bit half of the value, each
+// holding (status << 32 | half).  status = epoch << 2 | kind.
+template<typename T>
+struct Desc
+{
+  static constexpr int W = sizeof(T) / 4;
+  uint64_t w[W];
+};
+
+template<typename T>
+__device__ __forceinline__ void publish(Desc<T>* d, uint32_t status, T v)
+{
+  uint32_t h[Desc<T>::W];
+  __builtin_memcpy(h, &v, sizeof(T));
+#pragma unroll
+  for (int k = 0; k < Desc<T>::W; ++k)
+  {
+    atomic_store_agent(&d->w[k], ((uint64_t)status << 32) | h[k]);
+  }
+}
+
+// Returns the kind (0 if not yet published in this epoch) and the value.
+template<typename T>
+__device__ __forceinline__ uint32_t peek(const Desc<T>* d, uint32_t epoch_tag, T& v)
+{
+  uint32_t h[Desc<T>::W];
+  uint32_t st0 = 0;
+  bool same = true;
+#pragma unroll
+  for (int k = 0; k < Desc<T>::W; ++k)
+  {
+    const uint64_t w = atomic_load_agent(&d->w[k]);
+    const uint32_t st = (uint32_t)(w >> 32);
+    if (k == 0) st0 = st;
+    same = same && (st == st0);
+    h[k] = (uint32_t)w;
+  }
+  __builtin_memcpy(&v, h, sizeof(T));
+  if (!same || (st0 & ~3u) != epoch_tag) return 0u;
+  return st0 & 3u;
+}
+
+template<typename T, typename Op>
+__device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, uint32_t epoch_tag)
+{
+  const Op op{};
+  const int lane = lane_id();
+  T acc = Op::ident();
+  int64_t base = tile - 1;
+  while (true)
+  {
+    const int64_t t = base - lane;
+    T v = Op::ident();
+    uint32_t kind = kInc;
+    if (t >= 0)
+    {
+      kind = peek<T>(desc + t, epoch_tag, v);
+      while (ballot(kind == 0u) != 0ull)
+      {
+        __builtin_amdgcn_s_sleep(1);
+        if (kind == 0u) kind = peek<T>(desc + t, epoch_tag, v);
+      }
+    }
+    else
+    {
+      v = Op::ident();
+    }
+    const uint64_t inc = ballot(kind == kInc);
+    if (inc)
+    {
+      const int first = __builtin_ctzll(inc);
+      const T contrib = (lane <= first) ? v : Op::ident();
+      acc = op(wave_reduce(contrib, op), acc);
+      return acc;
+    }
+    acc = op(wave_reduce(v, op), acc);
+    base -= wave_size;
+  }
+}
+
+template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED>
+__global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64_t n, T init,
+                                                      Desc<T>* __restrict__ desc,
+                                                      LaunchControl* ctl)
+{
+  using S = ScanShape<T>;
+  using vec_t = typename S::vec_t;
+  constexpr int VEC = S::VEC;
+  constexpr int V = S::V;
+  const Op op{};
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_epoch;
+  __shared__ T s_wave_agg[kWaves];
+  __shared__ T s_prefix;
+
+  if (threadIdx.x == 0)
+  {
+    s_tile = atomic_add_agent(&ctl->ticket, 1u);
+    s_epoch = atomic_load_agent(&ctl->epoch);
+  }
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const uint32_t epoch_tag = (s_epoch & 0x3FFFFFFFu) << 2;
+  const int lane = lane_id();
+  const int wave = wave_id();
+  const int64_t wbase = tile * S::TILE + (int64_t)wave * S::WCHUNK;
+  const bool full = ALIGNED && (tile + 1) * S::TILE <= n;
+
+  T v[V][VEC];
+  if (full)
+  {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+    {
+      const vec_t x = *reinterpret_cast<const vec_t*>(in + wbase + ((int64_t)k * wave_size + lane) * VEC);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) v[k][e] = x[e];
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+    {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+      {
+        const int64_t i = wbase + ((int64_t)k * wave_size + lane) * VEC + e;
+        v[k][e] = (i < n) ? in[i] : Op::ident();
+      }
+    }
+  }
+
+  // Per-lane exclusive prefix of each vector within the wave's chunk.
+  T lane_pref[V];
+  T run = Op::ident();
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+  {
+    T t = v[k][0];
+#pragma unroll
+    for (int e = 1; e < VEC; ++e) t = op(t, v[k][e]);
+    T excl, total;
+    (void)wave_scan(t, op, Op::ident(), excl, total);
+    lane_pref[k] = op(run, excl);
+    run = op(run, total);
+  }
+  if (lane == 0) s_wave_agg[wave] = run;
+  __syncthreads();
+  T wave_pref = Op::ident();
+  T tile_agg = Op::ident();
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w)
+  {
+    if (w < wave) wave_pref = op(wave_pref, s_wave_agg[w]);
+    tile_agg = op(tile_agg, s_wave_agg[w]);
+  }
+  const T start = EXCLUSIVE ? init : Op::ident();
+  if (wave == 0)
+  {
+    T prefix;
+    if (tile == 0)
+    {
+      prefix = start;
+      if (lane == 0) publish<T>(desc + tile, epoch_tag | kInc, op(prefix, tile_agg));
+    }
+    else
+    {
+      if (lane == 0) publish<T>(desc + tile, epoch_tag | kAgg, tile_agg);
+      // Inclusive descriptors already hold `start` (tile 0 folds it in).
+      prefix = lookback<T, Op>(desc, tile, epoch_tag);
+      if (lane == 0) publish<T>(desc + tile, epoch_tag | kInc, op(prefix, tile_agg));
+    }
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  const T base_pref = op(s_prefix, wave_pref);
+
+  T o[V][VEC];
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+  {
+    T acc = op(base_pref, lane_pref[k]);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e)
+    {
+      if (EXCLUSIVE)
+      {
+        o[k][e] = acc;
+        acc = op(acc, v[k][e]);
+      }
+      else
+      {
+        acc = op(acc, v[k][e]);
+        o[k][e] = acc;
+      }
+    }
+  }
+  if (full)
+  {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+    {
+      vec_t x;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) x[e] = o[k][e];
+      *reinterpret_cast<vec_t*>(out + wbase + ((int64_t)k * wave_size + lane) * VEC) = x;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+    {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+      {
+        const int64_t i = wbase + ((int64_t)k * wave_size + lane) * VEC + e;
+        if (i < n) out[i] = o[k][e];
+      }
+    }
+  }
+  if (threadIdx.x == 0)
+  {
+    finish_tiles(ctl, gridDim.x, true);
+  }
+}
+
+template<typename T, typename Op, bool EXCLUSIVE>
+int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name)
+{
+  if (n < 0) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "negative length");
+  if (n == 0) return RAJA_CDNA4_SUCCESS;  // empty range: no-op (pattern/scan.hpp:72-75)
+  if (!in || !out) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
+  using S = ScanShape<T>;
+  const int64_t ntiles = rc::ceil_div(n, S::TILE);
+  if (ntiles > 0x7FFFFFFF) return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan too long");
+  rc::StreamCtx* c = nullptr;
+  RC_TRY(rc::get_ctx(s, &c));
+  RC_TRY(rc::ensure_scan_desc(c, (size_t)ntiles * sizeof(Desc<T>)));
+  const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
+  rc::TimedScope ts(name, s);
+  if (aligned)
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true>), dim3((unsigned)ntiles), dim3(kBlock), 0,
+                       s, in, out, n, init, (Desc<T>*)c->scan_desc, c->ctl);
+  }
+  else
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false>), dim3((unsigned)ntiles), dim3(kBlock),
+                       0, s, in, out, n, init, (Desc<T>*)c->scan_desc, c->ctl);
+  }
+  RC_HIP(hipGetLastError());
+  return RAJA_CDNA4_SUCCESS;
+}
+
+template<typename T, bool EXCLUSIVE>
+int scan_typed(int op, const void* in, void* out, int64_t n, const void* init, hipStream_t s)
+{
+  const char* nm = EXCLUSIVE ? "exclusive_scan" : "inclusive_scan";
+  switch (op)
+  {
+    case RAJA_CDNA4_OP_PLUS:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpPlus<T>::ident();
+      return launch_scan<T, rk::OpPlus<T>, EXCLUSIVE>((const T*)in, (T*)out, n, iv, s, nm);
+    }
+    case RAJA_CDNA4_OP_MIN:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpMin<T>::ident();
+      return launch_scan<T, rk::OpMin<T>, EXCLUSIVE>((const T*)in, (T*)out, n, iv, s, nm);
+    }
+    case RAJA_CDNA4_OP_MAX:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpMax<T>::ident();
+      return launch_scan<T, rk::OpMax<T>, EXCLUSIVE>((const T*)in, (T*)out, n, iv, s, nm);
+    }
+    default:
+      return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan operator %d not supported", op);
+  }
+}
+
+template<bool EXCLUSIVE>
+int scan_dispatch(int dtype, int op, const void* in, void* out, int64_t n, const void* init,
+                  hipStream_t s)
+{
+  switch (dtype)
+  {
+    case RAJA_CDNA4_I32: return scan_typed<int32_t, EXCLUSIVE>(op, in, out, n, init, s);
+    case RAJA_CDNA4_I64: return scan_typed<int64_t, EXCLUSIVE>(op, in, out, n, init, s);
+    case RAJA_CDNA4_U32: return scan_typed<uint32_t, EXCLUSIVE>(op, in, out, n, init, s);
+    case RAJA_CDNA4_U64: return scan_typed<uint64_t, EXCLUSIVE>(op, in, out, n, init, s);
+    case RAJA_CDNA4_F32: return scan_typed<float, EXCLUSIVE>(op, in, out, n, init, s);
+    case RAJA_CDNA4_F64: return scan_typed<double, EXCLUSIVE>(op, in, out, n, init, s);
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "dtype %d not supported", dtype);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int raja_cdna4_inclusive_scan(int dtype, int op, const void* in, void* out, int64_t n,
+                              void* stream)
+{
+  return scan_dispatch<false>(dtype, op, in, out, n, nullptr, (hipStream_t)stream);
+}
+
+int raja_cdna4_exclusive_scan(int dtype, int op, const void* in, void* out, int64_t n,
+                              const void* init, void* stream)
+{
+  return scan_dispatch<true>(dtype, op, in, out, n, init, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,427 @@
+// Onesweep LSD radix sort of keys or (key, value) pairs.
+//
+// Reference: RAJA's HIP backend calls rocprim::radix_sort_keys/pairs over
+// all key bits with double buffers and copies back when the result lands in
+// the alternate buffer (include/RAJA/policy/hip/sort.hpp:77-495, 653-689);
+// the semantics are RAJA's stable sort by operators::less / greater.
+//
+// Design (8-bit digits, P = 8*sizeof(key)/8 passes):
+//   1. one read of all keys builds the P digit histograms (LDS counters per
+//      block, wave-uniform digits added once per wave, then global atomics);
+//   2. the host reads the histograms back for large inputs and skips every
+//      pass whose digit is the same for all keys;
+//   3. each pass is one kernel: a block takes a tile id from a counter,
+//      ranks its keys inside each wave by 8 ballots (peer lanes with the same
+//      digit) against per-wave LDS digit counters, publishes its per-digit
+//      counts, looks back over predecessor tiles for each digit (one thread
+//      per digit), stages keys and values in LDS in digit order and writes
+//      them out so equal digits land in contiguous runs.
+// Ranking follows input order inside a wave (item j, then lane) and across
+// waves and tiles, so the sort is stable.
+#include "ops.hpp"
+#include "runtime.hpp"
+
+namespace
+{
+using namespace RAJA::cdna4::dev;
+
+constexpr int kBlock = 256;  // one thread per digit in the per-digit phases
+constexpr int kWaves = kBlock / wave_size;
+constexpr int kRadix = 256;
+constexpr uint32_t kAgg = 1u, kInc = 2u;
+constexpr uint32_t kCountMask = 0x3FFFFFFFu;
+constexpr int64_t kMaxSortN = (int64_t)kCountMask;  // counts live in 30 bits
+
+// Order-preserving map of a key to unsigned bits (operator< order).
+template<typename K>
+struct KeyTraits;
+template<>
+struct KeyTraits<uint32_t>
+{
+  using bits_t = uint32_t;
+  __device__ static bits_t to_bits(uint32_t k) { return k; }
+};
+template<>
+struct KeyTraits<int32_t>
+{
+  using bits_t = uint32_t;
+  __device__ static bits_t to_bits(int32_t k) { return (uint32_t)k ^ 0x80000000u; }
+};
+template<>
+struct KeyTraits<float>
+{
+  using bits_t = uint32_t;
+  __device__ static bits_t to_bits(float k)
+  {
+    const uint32_t u = __float_as_uint(k);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  }
+};
+template<>
+struct KeyTraits<uint64_t>
+{
+  using bits_t = uint64_t;
+  __device__ static bits_t to_bits(uint64_t k) { return k; }
+};
+template<>
+struct KeyTraits<int64_t>
+{
+  using bits_t = uint64_t;
+  __device__ static bits_t to_bits(int64_t k) { return (uint64_t)k ^ 0x8000000000000000ull; }
+};
+template<>
+struct KeyTraits<double>
+{
+  using bits_t = uint64_t;
+  __device__ static bits_t to_bits(double k)
+  {
+    const uint64_t u = (uint64_t)__double_as_longlong(k);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+  }
+};
+
+template<typename K, bool DESC>
+__device__ __forceinline__ uint32_t digit_of(K k, int shift)
+{
+  auto b = KeyTraits<K>::to_bits(k);
+  if (DESC) b = ~b;
+  return (uint32_t)(b >> shift) & (kRadix - 1);
+}
+
+__global__ void clear_words_kernel(uint32_t* p, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+  {
+    p[i] = 0u;
+  }
+}
+
+// ---- 1. all-pass digit histograms ------------------------------------------
+template<typename K, bool DESC, int P>
+__global__ __launch_bounds__(kBlock) void hist_kernel(const K* __restrict__ keys, int64_t n,
+                                                      uint32_t* __restrict__ hist,
+                                                      uint32_t* __restrict__ desc0,
+                                                      int64_t desc0_words)
+{
+  __shared__ uint32_t s_hist[P * kRadix];
+  for (int i = threadIdx.x; i < P * kRadix; i += kBlock) s_hist[i] = 0u;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  // The loop index is the wave's first element, so whole waves iterate
+  // together and the wave-uniform test sees every lane.
+  for (int64_t ws = (int64_t)blockIdx.x * kBlock + (int64_t)wave_id() * wave_size; ws < n;
+       ws += stride)
+  {
+    const int64_t i = ws + lane_id();
+    const bool valid = i < n;
+    const K k = valid ? keys[i] : K(0);
+    const uint64_t vmask = ballot(valid);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+    {
+      const uint32_t d = digit_of<K, DESC>(k, 8 * p);
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+      const bool uniform = (ballot(d == d0 || !valid) == ~0ull) && (vmask & 1ull);
+      if (uniform)
+      {
+        if (lane_id() == 0) atomicAdd(&s_hist[p * kRadix + d0], (uint32_t)__popcll(vmask));
+      }
+      else if (valid)
+      {
+        atomicAdd(&s_hist[p * kRadix + d], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P * kRadix; i += kBlock)
+  {
+    const uint32_t c = s_hist[i];
+    if (c) atomic_add_agent(&hist[i], c);
+  }
+  // Clear the descriptors the first pass will use.
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < desc0_words; i += stride)
+  {
+    desc0[i] = 0u;
+  }
+}
+
+// ---- 2. one onesweep pass -----------------------------------------------------
+template<typename K, typename V, bool DESC, bool HAS_VALS, int ITEMS>
+__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+    const K* __restrict__ keys_in, K* __restrict__ keys_out, const V* __restrict__ vals_in,
+    V* __restrict__ vals_out, int64_t n, int shift, const uint32_t* __restrict__ hist,
+    uint32_t* __restrict__ desc_cur, uint32_t* __restrict__ desc_next, LaunchControl* ctl)
+{
+  constexpr int TILE = kBlock * ITEMS;
+  constexpr int WCHUNK = wave_size * ITEMS;
+  __shared__ K s_keys[TILE];
+  __shared__ V s_vals[HAS_VALS ? TILE : 1];
+  __shared__ uint32_t s_whist[kWaves * kRadix];
+  __shared__ int32_t s_off[kRadix];
+  __shared__ uint32_t s_tstart[kRadix];
+  __shared__ uint32_t s_scan[2 * kWaves];
+  __shared__ uint32_t s_tile;
+
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = wave_id();
+  for (int i = tid; i < kWaves * kRadix; i += kBlock) s_whist[i] = 0u;
+  const int64_t tile = draw_tile(ctl, &s_tile);  // includes a __syncthreads
+  const int64_t tbase = tile * TILE;
+  const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
+
+  K k[ITEMS];
+  V v[HAS_VALS ? ITEMS : 1];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+  {
+    const int64_t i = wbase + (int64_t)j * wave_size + lane;
+    k[j] = (i < n) ? keys_in[i] : K(0);
+    if (HAS_VALS) v[j] = (i < n) ? vals_in[i] : V(0);
+  }
+  desc_next[tile * kRadix + tid] = 0u;  // rows for the next pass
+
+  // Wave-local ranks: rank = digit count so far in this wave + lower peers.
+  uint32_t rank[ITEMS];
+  uint8_t dig[ITEMS];
+  const uint64_t lt_mask = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+  {
+    const int64_t i = wbase + (int64_t)j * wave_size + lane;
+    const bool valid = i < n;
+    const uint32_t d = digit_of<K, DESC>(k[j], shift);
+    dig[j] = (uint8_t)d;
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+    {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const uint32_t lower = (uint32_t)__popcll(peers & lt_mask);
+    uint32_t cnt = 0;
+    if (valid) cnt = s_whist[wave * kRadix + d];
+    rank[j] = cnt + lower;
+    if (valid && lower == 0) s_whist[wave * kRadix + d] = cnt + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+
+  // Per digit (thread = digit): wave offsets, tile count, publication.
+  const int d = tid;
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w)
+  {
+    const uint32_t c = s_whist[w * kRadix + d];
+    s_whist[w * kRadix + d] = total;
+    total += c;
+  }
+  if (tile == 0)
+  {
+    atomic_store_agent(&desc_cur[d], (kInc << 30) | total);
+  }
+  else
+  {
+    atomic_store_agent(&desc_cur[tile * kRadix + d], (kAgg << 30) | total);
+  }
+  // Exclusive scans over digits of the tile counts and the global counts.
+  const uint32_t gcount = hist[d];
+  uint32_t ex_t, tot_t, ex_g, tot_g;
+  (void)wave_scan(total, rk::OpPlus<uint32_t>{}, 0u, ex_t, tot_t);
+  (void)wave_scan(gcount, rk::OpPlus<uint32_t>{}, 0u, ex_g, tot_g);
+  if (lane == 0)
+  {
+    s_scan[wave] = tot_t;
+    s_scan[kWaves + wave] = tot_g;
+  }
+  // Look back over predecessor tiles for this digit.
+  uint32_t prefix = 0;
+  if (tile > 0)
+  {
+    int64_t t = tile - 1;
+    while (true)
+    {
+      const uint32_t w = atomic_load_agent(&desc_cur[t * kRadix + d]);
+      const uint32_t kind = w >> 30;
+      if (kind == 0u)
+      {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      prefix += w & kCountMask;
+      if (kind == kInc) break;
+      --t;
+    }
+    atomic_store_agent(&desc_cur[tile * kRadix + d], (kInc << 30) | (prefix + total));
+  }
+  __syncthreads();
+  uint32_t tstart = ex_t, gstart = ex_g;
+  for (int w = 0; w < wave; ++w)
+  {
+    tstart += s_scan[w];
+    gstart += s_scan[kWaves + w];
+  }
+  s_tstart[d] = tstart;
+  s_off[d] = (int32_t)(gstart + prefix) - (int32_t)tstart;
+  __syncthreads();
+
+  // Stage in LDS in digit order.
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+  {
+    const int64_t i = wbase + (int64_t)j * wave_size + lane;
+    if (i < n)
+    {
+      const uint32_t dj = dig[j];
+      const uint32_t pos = s_tstart[dj] + s_whist[wave * kRadix + dj] + rank[j];
+      s_keys[pos] = k[j];
+      if (HAS_VALS) s_vals[pos] = v[j];
+    }
+  }
+  __syncthreads();
+  const int64_t rem = n - tbase;
+  const int tile_n = (rem < TILE) ? (int)rem : TILE;
+  for (int i = tid; i < tile_n; i += kBlock)
+  {
+    const K key = s_keys[i];
+    const int64_t dst = (int64_t)s_off[digit_of<K, DESC>(key, shift)] + i;
+    if (dst >= 0 && dst < n)  // always true for consistent histograms
+    {
+      keys_out[dst] = key;
+      if (HAS_VALS) vals_out[dst] = s_vals[i];
+    }
+  }
+  if (tid == 0) finish_tiles(ctl, gridDim.x, false);
+}
+
+struct Empty
+{
+};
+
+template<typename K, typename V, bool HAS_VALS, bool DESC>
+int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
+{
+  constexpr int P = (int)sizeof(K);
+  constexpr int ITEMS = (sizeof(K) == 4 && sizeof(V) <= 4) ? 16 : 8;
+  constexpr int64_t TILE = (int64_t)kBlock * ITEMS;
+  if (n > kMaxSortN)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort length %lld exceeds %lld",
+                         (long long)n, (long long)kMaxSortN);
+  }
+  const int64_t ntiles = rc::ceil_div(n, TILE);
+  rc::StreamCtx* c = nullptr;
+  RC_TRY(rc::get_ctx(s, &c));
+  const size_t keys_bytes = rc::align_up((size_t)n * sizeof(K), 256);
+  const size_t vals_bytes = HAS_VALS ? rc::align_up((size_t)n * sizeof(V), 256) : 0;
+  const size_t hist_bytes = (size_t)P * kRadix * sizeof(uint32_t);
+  RC_TRY(rc::ensure_tmp(c, keys_bytes + vals_bytes + hist_bytes));
+  const int64_t desc_words = ntiles * kRadix;
+  RC_TRY(rc::ensure_sort_desc(c, 2 * (size_t)desc_words * sizeof(uint32_t)));
+  char* tmp = (char*)c->tmp;
+  K* alt_keys = (K*)tmp;
+  V* alt_vals = HAS_VALS ? (V*)(tmp + keys_bytes) : nullptr;
+  uint32_t* hist = (uint32_t*)(tmp + keys_bytes + vals_bytes);
+  uint32_t* desc[2] = {(uint32_t*)c->sort_desc, (uint32_t*)c->sort_desc + desc_words};
+
+  {
+    rc::TimedScope ts("radix_hist", s);
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(kBlock), 0, s, hist, (int64_t)P * kRadix);
+    const int64_t want = rc::ceil_div(n, (int64_t)kBlock * 64);
+    const int64_t cap = (int64_t)rc::num_cus() * 4;
+    const unsigned grid = (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
+    hipLaunchKernelGGL((hist_kernel<K, DESC, P>), dim3(grid), dim3(kBlock), 0, s, keys, n, hist,
+                       desc[0], desc_words);
+    RC_HIP(hipGetLastError());
+  }
+  bool active[P];
+  for (int p = 0; p < P; ++p) active[p] = true;
+  if (n >= (int64_t)1 << 18)
+  {
+    uint32_t* h = (uint32_t*)c->pinned;
+    RC_HIP(hipMemcpyAsync(h, hist, hist_bytes, hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    for (int p = 0; p < P; ++p)
+    {
+      for (int dd = 0; dd < kRadix; ++dd)
+      {
+        if ((int64_t)h[p * kRadix + dd] == n) active[p] = false;
+      }
+    }
+  }
+  K* ksrc = keys;
+  K* kdst = alt_keys;
+  V* vsrc = vals;
+  V* vdst = alt_vals;
+  int q = 0;
+  for (int p = 0; p < P; ++p)
+  {
+    if (!active[p]) continue;
+    rc::TimedScope ts("onesweep_pass", s);
+    hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, ITEMS>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, ksrc, kdst, vsrc, vdst, n, 8 * p, hist + p * kRadix,
+                       desc[q & 1], desc[(q + 1) & 1], c->ctl);
+    RC_HIP(hipGetLastError());
+    K* tk = ksrc; ksrc = kdst; kdst = tk;
+    V* tv = vsrc; vsrc = vdst; vdst = tv;
+    ++q;
+  }
+  if (ksrc != keys)
+  {
+    RC_HIP(hipMemcpyAsync(keys, ksrc, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if (HAS_VALS)
+    {
+      RC_HIP(hipMemcpyAsync(vals, vsrc, (size_t)n * sizeof(V), hipMemcpyDeviceToDevice, s));
+    }
+  }
+  return RAJA_CDNA4_SUCCESS;
+}
+
+template<typename K>
+int sort_key_type(void* keys, void* vals, int val_bytes, int64_t n, int descending, hipStream_t s)
+{
+  if (!vals)
+  {
+    return descending ? sort_impl<K, uint32_t, false, true>((K*)keys, nullptr, n, s)
+                      : sort_impl<K, uint32_t, false, false>((K*)keys, nullptr, n, s);
+  }
+  if (val_bytes == 4)
+  {
+    return descending ? sort_impl<K, uint32_t, true, true>((K*)keys, (uint32_t*)vals, n, s)
+                      : sort_impl<K, uint32_t, true, false>((K*)keys, (uint32_t*)vals, n, s);
+  }
+  if (val_bytes == 8)
+  {
+    return descending ? sort_impl<K, uint64_t, true, true>((K*)keys, (uint64_t*)vals, n, s)
+                      : sort_impl<K, uint64_t, true, false>((K*)keys, (uint64_t*)vals, n, s);
+  }
+  return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "value size %d (4 or 8 supported)", val_bytes);
+}
+
+}  // namespace
+
+extern "C" {
+
+int raja_cdna4_sort_pairs(int key_dtype, void* keys, void* vals, int val_bytes, int64_t n,
+                          int descending, void* stream)
+{
+  if (n < 0) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "negative length");
+  if (n <= 1) return RAJA_CDNA4_SUCCESS;  // pattern/sort.hpp:74-82
+  if (!keys) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "keys is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  switch (key_dtype)
+  {
+    case RAJA_CDNA4_U32: return sort_key_type<uint32_t>(keys, vals, val_bytes, n, descending, s);
+    case RAJA_CDNA4_I32: return sort_key_type<int32_t>(keys, vals, val_bytes, n, descending, s);
+    case RAJA_CDNA4_F32: return sort_key_type<float>(keys, vals, val_bytes, n, descending, s);
+    case RAJA_CDNA4_U64: return sort_key_type<uint64_t>(keys, vals, val_bytes, n, descending, s);
+    case RAJA_CDNA4_I64: return sort_key_type<int64_t>(keys, vals, val_bytes, n, descending, s);
+    case RAJA_CDNA4_F64: return sort_key_type<double>(keys, vals, val_bytes, n, descending, s);
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "key dtype %d", key_dtype);
+  }
+}
+
+}  // extern "C"
