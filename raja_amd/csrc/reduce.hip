@@ -1,15 +1,19 @@
-// Deterministic single-pass span reductions (ReduceSum/Min/Max/BitOr/BitAnd,
+// Deterministic span reductions (ReduceSum/Min/Max/BitOr/BitAnd,
 // MinLoc/MaxLoc and the fused Sum+Min+Max+MinLoc of one forall).
 //
 // Reference: the HIP reducers fold per-thread values with a shuffle tree,
 // per-block slots and a last-block-done pass, replicated 32 ways and folded
 // again on the host (include/RAJA/policy/hip/reduce.hpp:127-208, 960-1161).
-// Here the span is cut into fixed tiles of TILE elements; each tile's value
-// is folded by one block in a fixed order (4 interleaved accumulators per
-// thread, DPP wave tree, waves in order) and stored; the last block to finish
-// folds the tile values in a fixed order and applies the caller's init.  The
-// combine tree therefore depends only on n, not on the grid, the timing or
-// whatever else shares the GPU, and no floating-point atomics are used.
+// Here the span is cut into fixed tiles of TILE elements.  Kernel 1 folds
+// each tile in a fixed order (4 interleaved accumulators per thread, DPP wave
+// tree, waves in order) and stores the tile value; kernel 2, one block, folds
+// the tile values in a fixed order and applies the caller's init.  The
+// combine tree depends only on n -- not on the grid, the timing or whatever
+// else shares the GPU -- and no floating-point atomics are used.  Two
+// kernels instead of a last-block-done pass: on MI355X a release fence at
+// device scope writes back the whole L2 of the block's XCD, and a
+// same-address device atomic per block serialises across the 8 XCDs; the
+// kernel boundary gives the visibility for free.
 #include "ops.hpp"
 #include "runtime.hpp"
 
@@ -18,7 +22,9 @@ namespace
 using namespace RAJA::cdna4::dev;
 
 constexpr int kBlock = 256;
-constexpr int kVecPerThread = 16;
+constexpr int kVecPerChunk = 16;  // 16-byte loads in flight per thread
+constexpr int kChunks = 4;        // chunks per tile
+constexpr int kFinalBlock = 1024;
 
 template<typename T>
 struct VecOf
@@ -30,10 +36,11 @@ struct VecOf
 template<typename T>
 constexpr int64_t tile_elems()
 {
-  return (int64_t)kBlock * kVecPerThread * VecOf<T>::n;
+  return (int64_t)kBlock * kVecPerChunk * kChunks * VecOf<T>::n;
 }
 
-// Fold of one tile by the whole block.  Returns the tile value in every thread.
+// Fold of one tile by the whole block.  Thread t owns vectors k*B + t of the
+// tile (k = 0..63), accumulated into acc[k % 4]; returns the tile value.
 template<typename T, typename Op, bool ALIGNED>
 __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x, int64_t n,
                                                         int64_t tile,
@@ -49,27 +56,31 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   acc_t acc[4] = {Op::ident(), Op::ident(), Op::ident(), Op::ident()};
   if (ALIGNED && base + TILE <= n)
   {
-    vec_t v[kVecPerThread];
-#pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k)
+    for (int c = 0; c < kChunks; ++c)
     {
-      v[k] = *reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC);
-    }
+      vec_t v[kVecPerChunk];
 #pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k)
-    {
-      const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e)
+      for (int j = 0; j < kVecPerChunk; ++j)
       {
-        acc[k & 3] = op(acc[k & 3], Op::lift(v[k][e], i0 + e));
+        const int k = c * kVecPerChunk + j;
+        v[j] = *reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC);
+      }
+#pragma unroll
+      for (int j = 0; j < kVecPerChunk; ++j)
+      {
+        const int k = c * kVecPerChunk + j;
+        const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+        {
+          acc[j & 3] = op(acc[j & 3], Op::lift(v[j][e], i0 + e));
+        }
       }
     }
   }
   else
   {
-#pragma unroll 4
-    for (int k = 0; k < kVecPerThread; ++k)
+    for (int k = 0; k < kVecPerChunk * kChunks; ++k)
     {
       const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
 #pragma unroll
@@ -88,14 +99,10 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
 
 template<typename T, typename Op, bool ALIGNED>
 __global__ __launch_bounds__(kBlock) void reduce_tiles_kernel(
-    const T* __restrict__ x, int64_t n, int64_t ntiles,
-    typename Op::acc_t* __restrict__ partials, LaunchControl* ctl,
-    typename Op::acc_t init, typename Op::acc_t* __restrict__ out)
+    const T* __restrict__ x, int64_t n, int64_t ntiles, typename Op::acc_t* __restrict__ partials)
 {
   using acc_t = typename Op::acc_t;
-  const Op op{};
   __shared__ acc_t s_scratch[max_waves];
-  __shared__ uint32_t s_last;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
   {
     acc_t v = fold_tile<T, Op, ALIGNED>(x, n, tile, s_scratch);
@@ -104,24 +111,23 @@ __global__ __launch_bounds__(kBlock) void reduce_tiles_kernel(
       partials[tile] = v;
     }
   }
-  // Last-block-done: publish this block's partials, count it finished.
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0)
-  {
-    const uint32_t prev = atomic_add_acq_rel_agent(&ctl->done, 1u);
-    s_last = (prev == gridDim.x - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // Fixed-order fold of the tile values: thread t takes tiles t, t+B, ...
-  // into 8 interleaved accumulators.
+}
+
+// Fixed-order fold of the tile values: thread t takes tiles t, t+1024, ...
+// into 8 interleaved accumulators, then the block tree.
+template<typename Op>
+__global__ __launch_bounds__(kFinalBlock) void reduce_final_kernel(
+    const typename Op::acc_t* __restrict__ partials, int64_t ntiles, typename Op::acc_t init,
+    typename Op::acc_t* __restrict__ out)
+{
+  using acc_t = typename Op::acc_t;
+  const Op op{};
+  __shared__ acc_t s_scratch[max_waves];
   acc_t acc[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) acc[j] = Op::ident();
   int j = 0;
-  for (int64_t k = threadIdx.x; k < ntiles; k += kBlock)
+  for (int64_t k = threadIdx.x; k < ntiles; k += kFinalBlock)
   {
     acc[j] = op(acc[j], partials[k]);
     j = (j + 1) & 7;
@@ -132,7 +138,6 @@ __global__ __launch_bounds__(kBlock) void reduce_tiles_kernel(
   if (threadIdx.x == 0)
   {
     *out = Op::with_init(init, a);
-    atomic_store_agent(&ctl->done, 0u);
   }
 }
 
@@ -151,20 +156,25 @@ int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
   const int64_t max_blocks = (int64_t)rc::num_cus() * 8;
   const unsigned grid = (unsigned)(ntiles < max_blocks ? ntiles : max_blocks);
   const bool aligned = (((uintptr_t)x) % 16) == 0;
+  acc_t* partials = (acc_t*)c->tmp;
   rc::TimedScope ts(name, s);
   if (aligned)
   {
     hipLaunchKernelGGL((reduce_tiles_kernel<T, Op, true>), dim3(grid), dim3(kBlock), 0, s, x, n,
-                       ntiles, (acc_t*)c->tmp, c->ctl, init, out);
+                       ntiles, partials);
   }
   else
   {
     hipLaunchKernelGGL((reduce_tiles_kernel<T, Op, false>), dim3(grid), dim3(kBlock), 0, s, x,
-                       n, ntiles, (acc_t*)c->tmp, c->ctl, init, out);
+                       n, ntiles, partials);
   }
+  RC_HIP(hipGetLastError());
+  hipLaunchKernelGGL((reduce_final_kernel<Op>), dim3(1), dim3(kFinalBlock), 0, s, partials, ntiles,
+                     init, out);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }
+
 
 template<typename T>
 int reduce_typed(int op, const void* x, int64_t n, const void* init, void* out, hipStream_t s)

@@ -54,6 +54,7 @@ struct StreamCtx
   size_t tmp_cap = 0;
   void* pinned = nullptr;        // 64 KiB pinned host staging
   void* small = nullptr;         // 64 KiB device slots for per-call results
+  uint32_t scan_epoch = 0;       // tag of the latest scan's lookback descriptors
 };
 
 constexpr size_t pinned_staging_bytes = 64 * 1024;

@@ -13,8 +13,11 @@
 //   * the tile aggregate is published at once, then wave 0 looks back over
 //     up to 64 predecessor tiles per step (one lane per tile) until it meets
 //     an inclusive prefix;
-//   * descriptors carry an epoch bumped by the previous scan kernel's last
-//     block, so they never need clearing (also under graph replay).
+//   * tile t is block t (no ticket counter: a same-address device atomic
+//     per block serialises across the 8 XCDs at ~11 ns each);
+//   * descriptors carry a per-stream epoch chosen by the host, so they never
+//     need clearing between calls; under graph capture (where the epoch is
+//     frozen into the graph) a clear kernel runs first.
 #include "ops.hpp"
 #include "runtime.hpp"
 
@@ -30,7 +33,7 @@ template<typename T>
 struct ScanShape
 {
   static constexpr int VEC = 16 / sizeof(T);
-  static constexpr int V = (sizeof(T) == 4) ? 4 : 8;  // vectors per lane
+  static constexpr int V = 8;                          // vectors per lane
   static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
   static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
@@ -116,29 +119,22 @@ __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, uint32_
   }
 }
 
+// Tile t is block t.  Workgroups are dispatched in increasing id order on
+// each XCD, so every tile a block waits for is already resident or done.
 template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED>
 __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64_t n, T init,
                                                       Desc<T>* __restrict__ desc,
-                                                      LaunchControl* ctl)
+                                                      uint32_t epoch_tag)
 {
   using S = ScanShape<T>;
   using vec_t = typename S::vec_t;
   constexpr int VEC = S::VEC;
   constexpr int V = S::V;
   const Op op{};
-  __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_epoch;
   __shared__ T s_wave_agg[kWaves];
   __shared__ T s_prefix;
 
-  if (threadIdx.x == 0)
-  {
-    s_tile = atomic_add_agent(&ctl->ticket, 1u);
-    s_epoch = atomic_load_agent(&ctl->epoch);
-  }
-  __syncthreads();
-  const int64_t tile = s_tile;
-  const uint32_t epoch_tag = (s_epoch & 0x3FFFFFFFu) << 2;
+  const int64_t tile = blockIdx.x;
   const int lane = lane_id();
   const int wave = wave_id();
   const int64_t wbase = tile * S::TILE + (int64_t)wave * S::WCHUNK;
@@ -258,9 +254,16 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
       }
     }
   }
-  if (threadIdx.x == 0)
+}
+
+// Zero lookback descriptors (used only when the stream is being captured:
+// a replayed graph reuses its epoch, so each replay starts from clean ones).
+__global__ void clear_desc_kernel(uint64_t* p, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
   {
-    finish_tiles(ctl, gridDim.x, true);
+    p[i] = 0ull;
   }
 }
 
@@ -277,16 +280,30 @@ int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const cha
   RC_TRY(rc::get_ctx(s, &c));
   RC_TRY(rc::ensure_scan_desc(c, (size_t)ntiles * sizeof(Desc<T>)));
   const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
+  // Descriptors carry the call's epoch; stale ones from earlier calls on this
+  // stream never match.  Under graph capture the epoch is frozen into the
+  // graph, so the descriptors are cleared by a kernel node instead.
+  c->scan_epoch = (c->scan_epoch + 1u) & 0x3FFFFFFFu;
+  if (c->scan_epoch == 0u) c->scan_epoch = 1u;
+  const uint32_t epoch_tag = c->scan_epoch << 2;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RC_HIP(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone)
+  {
+    const int64_t words = ntiles * (int64_t)sizeof(Desc<T>) / 8;
+    hipLaunchKernelGGL(clear_desc_kernel, dim3((unsigned)rc::ceil_div(words, 256 * 16) + 1),
+                       dim3(256), 0, s, (uint64_t*)c->scan_desc, words);
+  }
   rc::TimedScope ts(name, s);
   if (aligned)
   {
     hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true>), dim3((unsigned)ntiles), dim3(kBlock), 0,
-                       s, in, out, n, init, (Desc<T>*)c->scan_desc, c->ctl);
+                       s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag);
   }
   else
   {
     hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false>), dim3((unsigned)ntiles), dim3(kBlock),
-                       0, s, in, out, n, init, (Desc<T>*)c->scan_desc, c->ctl);
+                       0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag);
   }
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;

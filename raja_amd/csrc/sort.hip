@@ -151,7 +151,7 @@ template<typename K, typename V, bool DESC, bool HAS_VALS, int ITEMS>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K* __restrict__ keys_in, K* __restrict__ keys_out, const V* __restrict__ vals_in,
     V* __restrict__ vals_out, int64_t n, int shift, const uint32_t* __restrict__ hist,
-    uint32_t* __restrict__ desc_cur, uint32_t* __restrict__ desc_next, LaunchControl* ctl)
+    uint32_t* __restrict__ desc_cur, uint32_t* __restrict__ desc_next)
 {
   constexpr int TILE = kBlock * ITEMS;
   constexpr int WCHUNK = wave_size * ITEMS;
@@ -161,13 +161,15 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   __shared__ int32_t s_off[kRadix];
   __shared__ uint32_t s_tstart[kRadix];
   __shared__ uint32_t s_scan[2 * kWaves];
-  __shared__ uint32_t s_tile;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = wave_id();
   for (int i = tid; i < kWaves * kRadix; i += kBlock) s_whist[i] = 0u;
-  const int64_t tile = draw_tile(ctl, &s_tile);  // includes a __syncthreads
+  __syncthreads();
+  // Tile t is block t: workgroups are dispatched in increasing id order on
+  // each XCD, so every tile this block waits for is resident or done.
+  const int64_t tile = blockIdx.x;
   const int64_t tbase = tile * TILE;
   const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
 
@@ -294,7 +296,6 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
       if (HAS_VALS) vals_out[dst] = s_vals[i];
     }
   }
-  if (tid == 0) finish_tiles(ctl, gridDim.x, false);
 }
 
 struct Empty
@@ -363,7 +364,7 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
     rc::TimedScope ts("onesweep_pass", s);
     hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, ITEMS>), dim3((unsigned)ntiles),
                        dim3(kBlock), 0, s, ksrc, kdst, vsrc, vdst, n, 8 * p, hist + p * kRadix,
-                       desc[q & 1], desc[(q + 1) & 1], c->ctl);
+                       desc[q & 1], desc[(q + 1) & 1]);
     RC_HIP(hipGetLastError());
     K* tk = ksrc; ksrc = kdst; kdst = tk;
     V* tv = vsrc; vsrc = vdst; vdst = tv;

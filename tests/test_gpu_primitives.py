@@ -170,7 +170,7 @@ def test_scan_golden(raja, golden):
         assert np.array_equal(to_np(out)[idx], np.array(case["expect"]["val"], np.int64)), case
 
 
-SCAN_SIZES = [1, 2, 100, 4095, 4096, 4097, 65536 * 3 + 11, (1 << 22) + 1]
+SCAN_SIZES = [1, 2, 100, 4095, 4096, 4097, 8191, 8192, 8193, 65536 * 3 + 11, (1 << 22) + 1]
 
 
 @pytest.mark.parametrize("dtype", [torch.int32, torch.int64, torch.uint32, torch.uint64])
