@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ops", action="store_true", help="skip the secondary ops")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -128,6 +130,15 @@ def main():
     same = hk[:-1] == hk[1:]
     ok = ok and bool(np.all(hv[:-1][same] < hv[1:][same]))
     del hk0, hk, hv, same
+
+    if args.calib_copy:
+        # A plain copy of known size: 1 GiB read + 1 GiB written per launch.
+        src = torch.empty(1 << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ops = {}
     if not args.no_ops:

@@ -10,12 +10,14 @@
 //      block, wave-uniform digits added once per wave, then global atomics);
 //   2. the host reads the histograms back for large inputs and skips every
 //      pass whose digit is the same for all keys;
-//   3. each pass is one kernel: a block takes a tile id from a counter,
-//      ranks its keys inside each wave by 8 ballots (peer lanes with the same
-//      digit) against per-wave LDS digit counters, publishes its per-digit
-//      counts, looks back over predecessor tiles for each digit (one thread
-//      per digit), stages keys and values in LDS in digit order and writes
-//      them out so equal digits land in contiguous runs.
+//   3. each pass is one kernel: block t owns tile t, ranks its keys inside
+//      each wave by 8 ballots (peer lanes with the same digit) against
+//      per-wave LDS digit counters, publishes its per-digit counts, looks
+//      back over predecessor tiles for each digit (one thread per digit),
+//      stages keys and then values in LDS in digit order and writes them out
+//      so equal digits land in contiguous runs.  Each pass clears the
+//      descriptors of the next one (two alternating arrays), so no clearing
+//      launch or device counter is needed.
 // Ranking follows input order inside a wave (item j, then lane) and across
 // waves and tiles, so the sort is stable.
 #include "ops.hpp"
@@ -30,6 +32,8 @@ constexpr int kWaves = kBlock / wave_size;
 constexpr int kRadix = 256;
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 constexpr uint32_t kCountMask = 0x3FFFFFFFu;
+// A descriptor row: 256 digit words, the tile's hint word, padding.
+constexpr int kRowStride = kRadix + 8;
 constexpr int64_t kMaxSortN = (int64_t)kCountMask;  // counts live in 30 bits
 
 // Order-preserving map of a key to unsigned bits (operator< order).
@@ -98,77 +102,120 @@ __global__ void clear_words_kernel(uint32_t* p, int64_t n)
 }
 
 // ---- 1. all-pass digit histograms ------------------------------------------
+// Each wave streams 64 lanes x U x 16-byte vectors per iteration (U vector
+// loads in flight per lane) and counts every digit of every key in LDS; a
+// digit shared by the whole wave is added once.
 template<typename K, bool DESC, int P>
-__global__ __launch_bounds__(kBlock) void hist_kernel(const K* __restrict__ keys, int64_t n,
-                                                      uint32_t* __restrict__ hist,
-                                                      uint32_t* __restrict__ desc0,
-                                                      int64_t desc0_words)
+__global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, int64_t n,
+                                                   uint32_t* __restrict__ hist,
+                                                   uint32_t* __restrict__ desc0,
+                                                   int64_t desc0_words)
 {
+  constexpr int VEC = 16 / sizeof(K);
+  constexpr int U = 4;
+  constexpr int PER_WAVE = wave_size * U * VEC;
+  typedef K vec_t __attribute__((ext_vector_type(16 / sizeof(K))));
   __shared__ uint32_t s_hist[P * kRadix];
-  for (int i = threadIdx.x; i < P * kRadix; i += kBlock) s_hist[i] = 0u;
+  for (int i = threadIdx.x; i < P * kRadix; i += 256) s_hist[i] = 0u;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  // The loop index is the wave's first element, so whole waves iterate
-  // together and the wave-uniform test sees every lane.
-  for (int64_t ws = (int64_t)blockIdx.x * kBlock + (int64_t)wave_id() * wave_size; ws < n;
-       ws += stride)
+  const int lane = lane_id();
+  const bool aligned = ((uintptr_t)keys % 16) == 0;
+  const int64_t nwaves = (int64_t)gridDim.x * (256 / wave_size);
+  for (int64_t base = ((int64_t)blockIdx.x * (256 / wave_size) + wave_id()) * PER_WAVE; base < n;
+       base += nwaves * PER_WAVE)
   {
-    const int64_t i = ws + lane_id();
-    const bool valid = i < n;
-    const K k = valid ? keys[i] : K(0);
-    const uint64_t vmask = ballot(valid);
-#pragma unroll
-    for (int p = 0; p < P; ++p)
+    K k[U * VEC];
+    const bool full = aligned && base + PER_WAVE <= n;
+    if (full)
     {
-      const uint32_t d = digit_of<K, DESC>(k, 8 * p);
-      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-      const bool uniform = (ballot(d == d0 || !valid) == ~0ull) && (vmask & 1ull);
-      if (uniform)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
       {
-        if (lane_id() == 0) atomicAdd(&s_hist[p * kRadix + d0], (uint32_t)__popcll(vmask));
+        const vec_t x = *reinterpret_cast<const vec_t*>(keys + base + ((int64_t)u * wave_size + lane) * VEC);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) k[u * VEC + e] = x[e];
       }
-      else if (valid)
+    }
+    else
+    {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
       {
-        atomicAdd(&s_hist[p * kRadix + d], 1u);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+        {
+          const int64_t i = base + ((int64_t)u * wave_size + lane) * VEC + e;
+          k[u * VEC + e] = (i < n) ? keys[i] : K(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < U * VEC; ++q)
+    {
+      const int64_t i = base + ((int64_t)(q / VEC) * wave_size + lane) * VEC + (q % VEC);
+      const bool valid = full || i < n;
+      // valid lanes of one item form a prefix in lane order
+      const uint64_t vmask = ballot(valid);
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+      {
+        const uint32_t d = digit_of<K, DESC>(k[q], 8 * p);
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+        const bool uniform = (ballot(d == d0 || !valid) == ~0ull) && (vmask & 1ull);
+        if (uniform)
+        {
+          if (lane == 0) atomicAdd(&s_hist[p * kRadix + d0], (uint32_t)__popcll(vmask));
+        }
+        else if (valid)
+        {
+          atomicAdd(&s_hist[p * kRadix + d], 1u);
+        }
       }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < P * kRadix; i += kBlock)
+  for (int i = threadIdx.x; i < P * kRadix; i += 256)
   {
     const uint32_t c = s_hist[i];
     if (c) atomic_add_agent(&hist[i], c);
   }
   // Clear the descriptors the first pass will use.
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < desc0_words; i += stride)
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < desc0_words; i += stride)
   {
     desc0[i] = 0u;
   }
 }
 
 // ---- 2. one onesweep pass -----------------------------------------------------
-template<typename K, typename V, bool DESC, bool HAS_VALS, int ITEMS>
-__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+// Tile t is block t: workgroups are dispatched in increasing id order on each
+// XCD, so every tile a block waits for is resident or done.  Keys, then
+// values, go through one LDS staging buffer so a 8192-key tile fits in
+// ~42 KiB and the output is written in digit-contiguous runs.
+template<typename K, typename V, bool DESC, bool HAS_VALS, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const K* __restrict__ keys_in, K* __restrict__ keys_out, const V* __restrict__ vals_in,
     V* __restrict__ vals_out, int64_t n, int shift, const uint32_t* __restrict__ hist,
     uint32_t* __restrict__ desc_cur, uint32_t* __restrict__ desc_next)
 {
-  constexpr int TILE = kBlock * ITEMS;
+  static_assert(BLOCK >= kRadix && BLOCK % wave_size == 0, "one thread per digit");
+  constexpr int NW = BLOCK / wave_size;
+  constexpr int TILE = BLOCK * ITEMS;
   constexpr int WCHUNK = wave_size * ITEMS;
-  __shared__ K s_keys[TILE];
-  __shared__ V s_vals[HAS_VALS ? TILE : 1];
-  __shared__ uint32_t s_whist[kWaves * kRadix];
+  constexpr int SLOT = (HAS_VALS && sizeof(V) > sizeof(K)) ? sizeof(V) : sizeof(K);
+  __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * SLOT];
+  __shared__ uint32_t s_whist[NW * kRadix];
   __shared__ int32_t s_off[kRadix];
   __shared__ uint32_t s_tstart[kRadix];
-  __shared__ uint32_t s_scan[2 * kWaves];
+  __shared__ uint32_t s_scan[2 * (kRadix / wave_size)];
+  __shared__ int64_t s_hit;
+  K* s_keys = reinterpret_cast<K*>(s_buf);
+  V* s_vals = reinterpret_cast<V*>(s_buf);
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = wave_id();
-  for (int i = tid; i < kWaves * kRadix; i += kBlock) s_whist[i] = 0u;
-  __syncthreads();
-  // Tile t is block t: workgroups are dispatched in increasing id order on
-  // each XCD, so every tile this block waits for is resident or done.
+  for (int i = tid; i < NW * kRadix; i += BLOCK) s_whist[i] = 0u;
   const int64_t tile = blockIdx.x;
   const int64_t tbase = tile * TILE;
   const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
@@ -182,11 +229,11 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     k[j] = (i < n) ? keys_in[i] : K(0);
     if (HAS_VALS) v[j] = (i < n) ? vals_in[i] : V(0);
   }
-  desc_next[tile * kRadix + tid] = 0u;  // rows for the next pass
+  for (int i = tid; i < kRowStride; i += BLOCK) desc_next[tile * kRowStride + i] = 0u;  // next pass
+  __syncthreads();
 
   // Wave-local ranks: rank = digit count so far in this wave + lower peers.
   uint32_t rank[ITEMS];
-  uint8_t dig[ITEMS];
   const uint64_t lt_mask = lanemask_lt();
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j)
@@ -194,7 +241,6 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const int64_t i = wbase + (int64_t)j * wave_size + lane;
     const bool valid = i < n;
     const uint32_t d = digit_of<K, DESC>(k[j], shift);
-    dig[j] = (uint8_t)d;
     uint64_t peers = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b)
@@ -211,116 +257,186 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
   __syncthreads();
 
-  // Per digit (thread = digit): wave offsets, tile count, publication.
-  const int d = tid;
-  uint32_t total = 0;
+  // Per digit (thread = digit): wave offsets, tile count, publication and
+  // digit scans.
+  uint32_t ex_t = 0, ex_g = 0, prefix = 0, total = 0;
+  uint32_t* const row = desc_cur + tile * kRowStride;
+  if (tid < kRadix)
+  {
+    const int d = tid;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w)
-  {
-    const uint32_t c = s_whist[w * kRadix + d];
-    s_whist[w * kRadix + d] = total;
-    total += c;
+    for (int w = 0; w < NW; ++w)
+    {
+      const uint32_t c = s_whist[w * kRadix + d];
+      s_whist[w * kRadix + d] = total;
+      total += c;
+    }
+    atomic_store_agent(&row[d], ((tile == 0 ? kInc : kAgg) << 30) | total);
+    const uint32_t gcount = hist[d];
+    uint32_t tot_t, tot_g;
+    (void)wave_scan(total, rk::OpPlus<uint32_t>{}, 0u, ex_t, tot_t);
+    (void)wave_scan(gcount, rk::OpPlus<uint32_t>{}, 0u, ex_g, tot_g);
+    if (lane == 0)
+    {
+      s_scan[wave] = tot_t;
+      s_scan[kRadix / wave_size + wave] = tot_g;
+    }
   }
-  if (tile == 0)
-  {
-    atomic_store_agent(&desc_cur[d], (kInc << 30) | total);
-  }
-  else
-  {
-    atomic_store_agent(&desc_cur[tile * kRadix + d], (kAgg << 30) | total);
-  }
-  // Exclusive scans over digits of the tile counts and the global counts.
-  const uint32_t gcount = hist[d];
-  uint32_t ex_t, tot_t, ex_g, tot_g;
-  (void)wave_scan(total, rk::OpPlus<uint32_t>{}, 0u, ex_t, tot_t);
-  (void)wave_scan(gcount, rk::OpPlus<uint32_t>{}, 0u, ex_g, tot_g);
-  if (lane == 0)
-  {
-    s_scan[wave] = tot_t;
-    s_scan[kWaves + wave] = tot_g;
-  }
-  // Look back over predecessor tiles for this digit.
-  uint32_t prefix = 0;
   if (tile > 0)
   {
-    int64_t t = tile - 1;
-    while (true)
+    // Wave 0 finds the nearest predecessor whose hint says all of its
+    // inclusive counts are published (one load per lane, 64 tiles a step).
+    if (wave == 0)
     {
-      const uint32_t w = atomic_load_agent(&desc_cur[t * kRadix + d]);
-      const uint32_t kind = w >> 30;
-      if (kind == 0u)
+      int64_t top = tile - 1;
+      int64_t hit = -1;
+      while (true)
       {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
+        const int64_t t = top - lane;
+        const uint32_t h = (t < 0) ? 1u : atomic_load_agent(&desc_cur[t * kRowStride + kRadix]);
+        const uint64_t m = ballot(h != 0u);
+        if (m)
+        {
+          hit = top - __builtin_ctzll(m);
+          break;
+        }
+        top -= wave_size;
       }
-      prefix += w & kCountMask;
-      if (kind == kInc) break;
-      --t;
+      if (lane == 0) s_hit = hit;
     }
-    atomic_store_agent(&desc_cur[tile * kRadix + d], (kInc << 30) | (prefix + total));
+    __syncthreads();
+    if (tid < kRadix)
+    {
+      // Descriptors tile-1 .. hit are read with all loads in flight at once;
+      // the fold stops at the first inclusive count.  A descriptor whose kind
+      // is not yet visible is re-read (the hint carries no ordering).
+      const int d = tid;
+      const int64_t hit = s_hit;
+      bool done = false;
+      for (int64_t t0 = tile - 1; !done; t0 -= 8)
+      {
+        uint32_t w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+          const int64_t t = t0 - u;
+          w[u] = (t >= 0 && t >= hit) ? atomic_load_agent(&desc_cur[t * kRowStride + d])
+                                      : (kInc << 30);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+          if (done) break;
+          const int64_t t = t0 - u;
+          if (t < 0 || t < hit)
+          {
+            done = true;
+            break;
+          }
+          // The hinted tile itself must show its inclusive count.
+          while ((w[u] >> 30) == 0u || (t == hit && (w[u] >> 30) != kInc))
+          {
+            __builtin_amdgcn_s_sleep(1);
+            w[u] = atomic_load_agent(&desc_cur[t * kRowStride + d]);
+          }
+          prefix += w[u] & kCountMask;
+          if ((w[u] >> 30) == kInc) done = true;
+        }
+      }
+      atomic_store_agent(&row[d], (kInc << 30) | (prefix + total));
+    }
   }
   __syncthreads();
-  uint32_t tstart = ex_t, gstart = ex_g;
-  for (int w = 0; w < wave; ++w)
+  if (tid == 0) atomic_store_agent(&row[kRadix], 1u);  // hint: inclusive counts published
+  if (tid < kRadix)
   {
-    tstart += s_scan[w];
-    gstart += s_scan[kWaves + w];
+    const int d = tid;
+    uint32_t tstart = ex_t, gstart = ex_g;
+    for (int w = 0; w < wave; ++w)
+    {
+      tstart += s_scan[w];
+      gstart += s_scan[kRadix / wave_size + w];
+    }
+    s_tstart[d] = tstart;
+    s_off[d] = (int32_t)(gstart + prefix) - (int32_t)tstart;
   }
-  s_tstart[d] = tstart;
-  s_off[d] = (int32_t)(gstart + prefix) - (int32_t)tstart;
   __syncthreads();
 
-  // Stage in LDS in digit order.
+  // Keys: stage in LDS in digit order, then write digit-contiguous runs.
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j)
   {
     const int64_t i = wbase + (int64_t)j * wave_size + lane;
     if (i < n)
     {
-      const uint32_t dj = dig[j];
-      const uint32_t pos = s_tstart[dj] + s_whist[wave * kRadix + dj] + rank[j];
-      s_keys[pos] = k[j];
-      if (HAS_VALS) s_vals[pos] = v[j];
+      const uint32_t dj = digit_of<K, DESC>(k[j], shift);
+      rank[j] += s_tstart[dj] + s_whist[wave * kRadix + dj];  // tile position
+      s_keys[rank[j]] = k[j];
     }
   }
   __syncthreads();
   const int64_t rem = n - tbase;
   const int tile_n = (rem < TILE) ? (int)rem : TILE;
-  for (int i = tid; i < tile_n; i += kBlock)
+  int32_t dst[HAS_VALS ? ITEMS : 1];
+#pragma unroll
+  for (int m = 0; m < ITEMS; ++m)
   {
-    const K key = s_keys[i];
-    const int64_t dst = (int64_t)s_off[digit_of<K, DESC>(key, shift)] + i;
-    if (dst >= 0 && dst < n)  // always true for consistent histograms
+    const int i = tid + m * BLOCK;
+    int32_t d = -1;
+    if (i < tile_n)
     {
-      keys_out[dst] = key;
-      if (HAS_VALS) vals_out[dst] = s_vals[i];
+      const K key = s_keys[i];
+      d = s_off[digit_of<K, DESC>(key, shift)] + i;
+      if (d >= 0 && (int64_t)d < n) keys_out[d] = key;  // always true for consistent histograms
+    }
+    if (HAS_VALS) dst[m] = d;
+  }
+  if (HAS_VALS)
+  {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+    {
+      const int64_t i = wbase + (int64_t)j * wave_size + lane;
+      if (i < n) s_vals[rank[j]] = v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < ITEMS; ++m)
+    {
+      const int i = tid + m * BLOCK;
+      if (i < tile_n && dst[m] >= 0 && (int64_t)dst[m] < n) vals_out[dst[m]] = s_vals[i];
     }
   }
 }
 
-struct Empty
+template<typename K, typename V, bool HAS_VALS>
+struct SortShape
 {
+  static constexpr bool wide = sizeof(K) > 4 || (HAS_VALS && sizeof(V) > 4);
+  static constexpr int BLOCK = wide ? 256 : 512;
+  static constexpr int ITEMS = 16;
+  static constexpr int64_t TILE = (int64_t)BLOCK * ITEMS;
 };
 
 template<typename K, typename V, bool HAS_VALS, bool DESC>
 int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
 {
+  using S = SortShape<K, V, HAS_VALS>;
   constexpr int P = (int)sizeof(K);
-  constexpr int ITEMS = (sizeof(K) == 4 && sizeof(V) <= 4) ? 16 : 8;
-  constexpr int64_t TILE = (int64_t)kBlock * ITEMS;
   if (n > kMaxSortN)
   {
     return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort length %lld exceeds %lld",
                          (long long)n, (long long)kMaxSortN);
   }
-  const int64_t ntiles = rc::ceil_div(n, TILE);
+  const int64_t ntiles = rc::ceil_div(n, S::TILE);
   rc::StreamCtx* c = nullptr;
   RC_TRY(rc::get_ctx(s, &c));
   const size_t keys_bytes = rc::align_up((size_t)n * sizeof(K), 256);
   const size_t vals_bytes = HAS_VALS ? rc::align_up((size_t)n * sizeof(V), 256) : 0;
   const size_t hist_bytes = (size_t)P * kRadix * sizeof(uint32_t);
   RC_TRY(rc::ensure_tmp(c, keys_bytes + vals_bytes + hist_bytes));
-  const int64_t desc_words = ntiles * kRadix;
+  const int64_t desc_words = ntiles * kRowStride;
   RC_TRY(rc::ensure_sort_desc(c, 2 * (size_t)desc_words * sizeof(uint32_t)));
   char* tmp = (char*)c->tmp;
   K* alt_keys = (K*)tmp;
@@ -330,11 +446,11 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
 
   {
     rc::TimedScope ts("radix_hist", s);
-    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(kBlock), 0, s, hist, (int64_t)P * kRadix);
-    const int64_t want = rc::ceil_div(n, (int64_t)kBlock * 64);
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(256), 0, s, hist, (int64_t)P * kRadix);
+    const int64_t want = rc::ceil_div(n, (int64_t)256 * 64);
     const int64_t cap = (int64_t)rc::num_cus() * 4;
     const unsigned grid = (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
-    hipLaunchKernelGGL((hist_kernel<K, DESC, P>), dim3(grid), dim3(kBlock), 0, s, keys, n, hist,
+    hipLaunchKernelGGL((hist_kernel<K, DESC, P>), dim3(grid), dim3(256), 0, s, keys, n, hist,
                        desc[0], desc_words);
     RC_HIP(hipGetLastError());
   }
@@ -362,9 +478,9 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
   {
     if (!active[p]) continue;
     rc::TimedScope ts("onesweep_pass", s);
-    hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, ITEMS>), dim3((unsigned)ntiles),
-                       dim3(kBlock), 0, s, ksrc, kdst, vsrc, vdst, n, 8 * p, hist + p * kRadix,
-                       desc[q & 1], desc[(q + 1) & 1]);
+    hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::ITEMS>),
+                       dim3((unsigned)ntiles), dim3(S::BLOCK), 0, s, ksrc, kdst, vsrc, vdst, n, 8 * p,
+                       hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
     RC_HIP(hipGetLastError());
     K* tk = ksrc; ksrc = kdst; kdst = tk;
     V* tv = vsrc; vsrc = vdst; vdst = tv;
