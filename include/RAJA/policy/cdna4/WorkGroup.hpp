@@ -1,0 +1,298 @@
+// cdna4 WorkGroup runners.
+//
+// Reference: include/RAJA/policy/hip/WorkGroup/WorkRunner.hpp:38-425 and
+// Dispatcher.hpp:42-115.  The reference's unordered HIP runner launches one
+// kernel with blockIdx.y selecting a loop and x sized by the *average* loop
+// length, so long loops are under-parallelised and short ones leave blocks
+// idle.  Here the loops are cut into fixed tiles of 4*B iterations listed in
+// a device tile table, built once per WorkGroup; one kernel runs one block
+// per tile, so 131072-iteration faces and 8-iteration corners share the GPU
+// evenly.  Ordered runs launch one kernel per loop in (reverse) enqueue
+// order, as the reference does.
+//
+// Dispatch: direct_dispatch<camp::list<Seg, Body>...> switches on the body's
+// position in the list at compile time (bodies inlined); the indirect
+// dispatch policies call through a device function pointer fetched once per
+// body type.
+#pragma once
+
+#include <vector>
+
+#include "RAJA/pattern/WorkGroup.hpp"
+#include "RAJA/policy/cdna4/forall.hpp"
+
+namespace RAJA
+{
+namespace policy
+{
+namespace cdna4
+{
+
+template<int BLOCK_SIZE, bool ASYNC = false>
+struct cdna4_work_t
+{
+  static_assert(BLOCK_SIZE % 64 == 0 && BLOCK_SIZE <= 1024, "block size: multiple of 64, <= 1024");
+  using resource_type = resources::Cdna4;
+  static constexpr int block_size = BLOCK_SIZE;
+  static constexpr bool async = ASYNC;
+};
+
+// Unordered runner over the tile table.
+struct unordered_cdna4_loop_tiles
+{
+};
+
+}  // namespace cdna4
+}  // namespace policy
+
+template<int B>
+using cdna4_work = policy::cdna4::cdna4_work_t<B, false>;
+template<int B>
+using cdna4_work_async = policy::cdna4::cdna4_work_t<B, true>;
+using policy::cdna4::unordered_cdna4_loop_tiles;
+// Name of the reference's unordered HIP runner, mapped to the tile runner.
+using unordered_cdna4_loop_y_block_iter_x_threadblock_average = unordered_cdna4_loop_tiles;
+
+namespace detail
+{
+namespace workgroup
+{
+
+template<int B, bool A>
+struct copies_bodies_to_device<policy::cdna4::cdna4_work_t<B, A>> : std::true_type
+{
+};
+
+template<typename Body, typename Index, typename... Args>
+__device__ void device_call(const void* body, Index i, Args... args)
+{
+  (*static_cast<const Body*>(body))(i, args...);
+}
+
+template<typename Index, typename... Args>
+using device_call_t = void (*)(const void*, Index, Args...);
+
+template<typename Body, typename Index, typename... Args>
+__global__ void device_fn_kernel(device_call_t<Index, Args...>* out)
+{
+  *out = &device_call<Body, Index, Args...>;
+}
+
+template<typename Body, typename Index, typename... Args>
+void* device_fn_of()
+{
+  static void* cached = nullptr;
+  if (!cached)
+  {
+    device_call_t<Index, Args...>* d = nullptr;
+    resources::detail::check_lib(raja_cdna4_device_malloc((void**)&d, sizeof(*d)), "fn slot");
+    hipLaunchKernelGGL((device_fn_kernel<Body, Index, Args...>), dim3(1), dim3(1), 0, 0, d);
+    resources::detail::check_hip(hipGetLastError(), "device_fn_kernel");
+    device_call_t<Index, Args...> h = nullptr;
+    resources::detail::check_hip(hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost), "fn copy");
+    resources::detail::check_lib(raja_cdna4_device_free(d), "fn slot");
+    cached = (void*)h;
+  }
+  return cached;
+}
+
+// ---- device dispatch -----------------------------------------------------------
+
+template<typename List, typename Index, typename... Args>
+struct DirectCall;
+template<typename Index, typename... Args>
+struct DirectCall<camp::list<>, Index, Args...>
+{
+  __device__ static void call(uint32_t, const void*, Index, Args...) {}
+};
+template<typename Seg, typename Body, typename... Rest, typename Index, typename... Args>
+struct DirectCall<camp::list<camp::list<Seg, Body>, Rest...>, Index, Args...>
+{
+  __device__ static void call(uint32_t k, const void* body, Index i, Args... args)
+  {
+    if (k == 0)
+    {
+      (*static_cast<const Body*>(body))(i, args...);
+    }
+    else
+    {
+      DirectCall<camp::list<Rest...>, Index, Args...>::call(k - 1, body, i, args...);
+    }
+  }
+};
+
+template<typename Dispatch, typename Index, typename... Args>
+struct Dispatcher
+{
+  __device__ static void call(const LoopRecord& L, const void* body, Index i, Args... args)
+  {
+    reinterpret_cast<device_call_t<Index, Args...>>(L.device_fn)(body, i, args...);
+  }
+};
+template<typename... RC, typename Index, typename... Args>
+struct Dispatcher<direct_dispatch<RC...>, Index, Args...>
+{
+  __device__ static void call(const LoopRecord& L, const void* body, Index i, Args... args)
+  {
+    DirectCall<camp::list<RC...>, Index, Args...>::call(L.type_index, body, i, args...);
+  }
+};
+
+struct Tile
+{
+  uint32_t loop;
+  uint32_t pad;
+  int64_t start;
+  int64_t end;
+};
+
+template<int B, typename Dispatch, typename Index, typename... Args>
+__global__ __launch_bounds__(B) void tiles_kernel(const Tile* __restrict__ tiles,
+                                                  const LoopRecord* __restrict__ loops,
+                                                  const unsigned char* __restrict__ bytes,
+                                                  Args... args)
+{
+  const Tile t = tiles[blockIdx.x];
+  const LoopRecord L = loops[t.loop];
+  const void* body = bytes + L.body_offset;
+  for (int64_t k = t.start + threadIdx.x; k < t.end; k += B)
+  {
+    Dispatcher<Dispatch, Index, Args...>::call(L, body, Index(L.begin + k), args...);
+  }
+}
+
+template<int B, typename Dispatch, typename Index, typename... Args>
+__global__ __launch_bounds__(B) void loop_kernel(const LoopRecord* __restrict__ loops,
+                                                 uint32_t l,
+                                                 const unsigned char* __restrict__ bytes,
+                                                 Args... args)
+{
+  const LoopRecord L = loops[l];
+  const void* body = bytes + L.body_offset;
+  const int64_t stride = (int64_t)gridDim.x * B;
+  for (int64_t k = (int64_t)blockIdx.x * B + threadIdx.x; k < L.len; k += stride)
+  {
+    Dispatcher<Dispatch, Index, Args...>::call(L, body, Index(L.begin + k), args...);
+  }
+}
+
+template<int B, bool A>
+struct RunData<policy::cdna4::cdna4_work_t<B, A>>
+{
+  unsigned char* d_bytes = nullptr;
+  LoopRecord* d_loops = nullptr;
+  Tile* d_tiles = nullptr;
+  int64_t ntiles = 0;
+  bool built = false;
+  hipStream_t stream = nullptr;
+
+  RunData() = default;
+  RunData(RunData&& o) noexcept { *this = std::move(o); }
+  RunData& operator=(RunData&& o) noexcept
+  {
+    std::swap(d_bytes, o.d_bytes);
+    std::swap(d_loops, o.d_loops);
+    std::swap(d_tiles, o.d_tiles);
+    std::swap(ntiles, o.ntiles);
+    std::swap(built, o.built);
+    std::swap(stream, o.stream);
+    return *this;
+  }
+
+  void release()
+  {
+    if (!built) return;
+    (void)hipStreamSynchronize(stream);
+    (void)raja_cdna4_device_free(d_bytes);
+    (void)raja_cdna4_device_free(d_loops);
+    if (d_tiles) (void)raja_cdna4_device_free(d_tiles);
+    d_bytes = nullptr;
+    d_loops = nullptr;
+    d_tiles = nullptr;
+    built = false;
+  }
+
+  // Upload the loop bodies, records and (for unordered runs) the tile table.
+  template<typename Storage>
+  void build(Storage& s, resources::Cdna4 r, bool tiles, bool indirect)
+  {
+    stream = r.get_stream();
+    std::vector<LoopRecord> recs(s.loops);
+    if (indirect)
+    {
+      for (size_t l = 0; l < recs.size(); ++l) recs[l].device_fn = s.host_fns[l].device_fn();
+    }
+    resources::detail::check_lib(raja_cdna4_device_malloc((void**)&d_bytes, s.used + 16), "wg bytes");
+    resources::detail::check_lib(
+        raja_cdna4_device_malloc((void**)&d_loops, sizeof(LoopRecord) * recs.size()), "wg loops");
+    r.memcpy(d_bytes, s.bytes.get(), s.used);
+    r.memcpy(d_loops, recs.data(), sizeof(LoopRecord) * recs.size());
+    if (tiles)
+    {
+      constexpr int64_t TILE = 4 * B;
+      std::vector<Tile> t;
+      for (size_t l = 0; l < recs.size(); ++l)
+      {
+        for (int64_t s0 = 0; s0 < recs[l].len; s0 += TILE)
+        {
+          t.push_back(Tile{(uint32_t)l, 0u, s0, std::min(recs[l].len, s0 + TILE)});
+        }
+      }
+      ntiles = (int64_t)t.size();
+      resources::detail::check_lib(raja_cdna4_device_malloc((void**)&d_tiles, sizeof(Tile) * t.size()),
+                                   "wg tiles");
+      r.memcpy(d_tiles, t.data(), sizeof(Tile) * t.size());
+    }
+    r.wait();  // host staging vectors go out of scope
+    built = true;
+  }
+};
+
+template<int B, bool A, typename Order, typename Dispatch, typename Index, typename... Args>
+struct WorkRunner<policy::cdna4::cdna4_work_t<B, A>, Order, Dispatch, Index, Args...>
+{
+  using resource_type = resources::Cdna4;
+  static constexpr bool unordered = !std::is_same<Order, ordered>::value &&
+                                    !std::is_same<Order, reverse_ordered>::value;
+  static constexpr bool indirect = std::is_same<typename dispatch_list<Dispatch>::type, void>::value;
+
+  template<typename Storage>
+  static void run(Storage& s, RunData<policy::cdna4::cdna4_work_t<B, A>>& rd, resource_type r,
+                  Args... args)
+  {
+    if (!rd.built) rd.build(s, r, unordered, indirect);
+    hipStream_t st = r.get_stream();
+    if (unordered)
+    {
+      if (rd.ntiles > 0)
+      {
+        hipLaunchKernelGGL((tiles_kernel<B, Dispatch, Index, Args...>), dim3((unsigned)rd.ntiles),
+                           dim3(B), 0, st, rd.d_tiles, rd.d_loops, rd.d_bytes, args...);
+        resources::detail::check_hip(hipGetLastError(), "workgroup tiles launch");
+      }
+    }
+    else
+    {
+      static int cap = 0;
+      if (cap == 0)
+      {
+        cap = RAJA::cdna4::detail::occupancy_grid(loop_kernel<B, Dispatch, Index, Args...>, B);
+      }
+      const size_t n = s.size();
+      for (size_t k = 0; k < n; ++k)
+      {
+        const size_t l = std::is_same<Order, reverse_ordered>::value ? n - 1 - k : k;
+        int64_t grid = (s.loops[l].len + B - 1) / B;
+        if (grid > cap) grid = cap;
+        hipLaunchKernelGGL((loop_kernel<B, Dispatch, Index, Args...>), dim3((unsigned)grid), dim3(B),
+                           0, st, rd.d_loops, (uint32_t)l, rd.d_bytes, args...);
+      }
+      resources::detail::check_hip(hipGetLastError(), "workgroup loop launch");
+    }
+    if (!A) r.wait();
+  }
+};
+
+}  // namespace workgroup
+}  // namespace detail
+}  // namespace RAJA

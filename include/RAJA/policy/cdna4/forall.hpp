@@ -1,0 +1,389 @@
+// cdna4 policy family: forall and reducers on an MI355X (gfx950).
+//
+// Reference counterparts: include/RAJA/policy/hip/policy.hpp:346-359,
+// 1246-1358 (hip_exec policies), include/RAJA/policy/hip/forall.hpp:365-639
+// (forall kernel and forall_impl), include/RAJA/policy/hip/reduce.hpp:
+// 960-1370 (reducers) and include/RAJA/policy/hip/params/reduce.hpp:19-60
+// (expt parameter reductions).
+//
+// Design:
+//   * one grid-stride kernel per loop body; the grid is sized from the
+//     occupancy calculator to fill the 256 CUs (or fixed by
+//     cdna4_exec_grid), never larger than the range needs;
+//   * the kernel makes a private copy of the body; reducers captured in it
+//     accumulate in registers and, when that copy is destroyed at kernel
+//     end, fold the block's threads with a fixed tree (DPP inside waves, LDS
+//     across waves) and store one value per block in pinned host memory;
+//   * get() (or, for expt parameters, the end of forall) waits for the
+//     stream and folds the block values on the host in block order: the
+//     result depends only on the range and the launch shape; no device
+//     atomics, no fences;
+//   * MinLoc/MaxLoc folds break value ties by the lower index, which is
+//     seq_exec's first-index result for increasing iteration order.
+#pragma once
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "RAJA/config.hpp"
+#include "RAJA/index/RangeSegment.hpp"
+#include "RAJA/pattern/params.hpp"
+#include "RAJA/pattern/reduce.hpp"
+#include "RAJA/policy/PolicyBase.hpp"
+#include "RAJA/policy/cdna4/intrinsics.hpp"
+#include "RAJA/util/resource.hpp"
+#include "raja_cdna4.h"
+
+namespace RAJA
+{
+namespace policy
+{
+namespace cdna4
+{
+
+template<int BLOCK_SIZE, bool ASYNC = false, int GRID_SIZE = 0>
+struct cdna4_exec_t
+{
+  static_assert(BLOCK_SIZE % 64 == 0 && BLOCK_SIZE <= 1024, "block size: multiple of 64, <= 1024");
+  static constexpr Policy policy = Policy::cdna4;
+  using resource_type = resources::Cdna4;
+  static constexpr int block_size = BLOCK_SIZE;
+  static constexpr bool async = ASYNC;
+  static constexpr int grid_size = GRID_SIZE;
+};
+
+struct cdna4_reduce
+{
+};
+
+struct cdna4_synchronize
+{
+};
+
+}  // namespace cdna4
+}  // namespace policy
+
+template<int B>
+using cdna4_exec = policy::cdna4::cdna4_exec_t<B, false, 0>;
+template<int B>
+using cdna4_exec_async = policy::cdna4::cdna4_exec_t<B, true, 0>;
+template<int B, int G, bool Async = false>
+using cdna4_exec_grid = policy::cdna4::cdna4_exec_t<B, Async, G>;
+template<int B>
+using cdna4_exec_occ_calc = policy::cdna4::cdna4_exec_t<B, false, 0>;
+using policy::cdna4::cdna4_reduce;
+using policy::cdna4::cdna4_synchronize;
+
+namespace cdna4
+{
+namespace detail
+{
+
+// Launch context for reducer registration: while it is set, a host-side copy
+// of a reducer is the copy that travels to the kernel and gets block slots.
+struct LaunchInfo
+{
+  bool setup = false;
+  int nblocks = 0;
+  hipStream_t stream = nullptr;
+};
+
+inline LaunchInfo& launch_info()
+{
+  static thread_local LaunchInfo info;
+  return info;
+}
+
+// Parallel fold operator of a reducer operator (ValueLoc ties go to the
+// lower index).
+template<typename Op>
+using tree_of = reduce::detail::tree_of<Op>;
+
+// Block fold at kernel end; thread 0 stores the block value.  Every thread
+// of the block must call it (it synchronises the block).
+template<typename T, typename TreeOp>
+__device__ inline void store_block_value(T v, T* slots)
+{
+  static_assert(sizeof(T) % 4 == 0, "cdna4 reducers need 4-byte multiple types");
+  // raw LDS bytes: __shared__ objects cannot have constructors
+  __shared__ __attribute__((aligned(16))) unsigned char s_raw[sizeof(T) * RAJA::cdna4::dev::max_waves];
+  const T r = RAJA::cdna4::dev::block_reduce(v, TreeOp{}, reinterpret_cast<T*>(s_raw));
+  if (threadIdx.x == 0) slots[blockIdx.x] = r;
+}
+
+inline void check(int status, const char* what)
+{
+  resources::detail::check_lib(status, what);
+}
+
+template<typename T>
+T* acquire_slots(int n)
+{
+  void* p = nullptr;
+  check(raja_cdna4_pinned_malloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1)), "reducer slots");
+  return static_cast<T*>(p);
+}
+
+inline void release_slots(void* p)
+{
+  if (p) check(raja_cdna4_pinned_free(p), "reducer slots");
+}
+
+}  // namespace detail
+
+// Storage of an old-style reducer for cdna4_reduce.
+template<typename T, typename Op>
+class Cdna4ReduceImpl
+{
+  using Tree = typename detail::tree_of<Op>::type;
+  struct Launch
+  {
+    T* slots;
+    int nblocks;
+    hipStream_t stream;
+  };
+  struct State
+  {
+    T value;  // init folded with every completed launch
+    T identity;
+    std::vector<Launch> launches;
+    std::mutex lock;
+  };
+
+public:
+  using value_type = T;
+
+  Cdna4ReduceImpl(T init, T identity)
+      : m_state(new State()), m_identity(identity), m_val(identity), m_owner(true)
+  {
+    m_state->value = init;
+    m_state->identity = identity;
+  }
+
+  RAJA_HOST_DEVICE Cdna4ReduceImpl(const Cdna4ReduceImpl& o)
+      : m_state(o.m_state), m_slots(o.m_slots), m_identity(o.m_identity), m_val(o.m_identity),
+        m_owner(false)
+  {
+#if defined(__HIP_DEVICE_COMPILE__)
+    m_device_private = true;
+#else
+    detail::LaunchInfo& li = detail::launch_info();
+    if (li.setup)
+    {
+      m_slots = detail::acquire_slots<T>(li.nblocks);
+      std::lock_guard<std::mutex> g(m_state->lock);
+      m_state->launches.push_back(Launch{m_slots, li.nblocks, li.stream});
+    }
+#endif
+  }
+
+  Cdna4ReduceImpl& operator=(const Cdna4ReduceImpl&) = delete;
+
+  RAJA_HOST_DEVICE ~Cdna4ReduceImpl()
+  {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (m_device_private && m_slots)
+    {
+      detail::store_block_value<T, Tree>(m_val, m_slots);
+    }
+#else
+    if (m_owner)
+    {
+      try
+      {
+        fold_launches();
+      }
+      catch (...)
+      {
+      }
+      delete m_state;
+    }
+#endif
+  }
+
+  RAJA_HOST_DEVICE void combine(const T& v) const
+  {
+#if defined(__HIP_DEVICE_COMPILE__)
+    m_val = Op{}(m_val, v);
+#else
+    m_state->value = Op{}(m_state->value, v);
+#endif
+  }
+
+  T get() const
+  {
+    fold_launches();
+    return m_state->value;
+  }
+
+  void reset(T init, T identity)
+  {
+    fold_launches();
+    m_state->value = init;
+    m_state->identity = identity;
+    m_identity = identity;
+  }
+
+private:
+  // Fold each launch's block values in block order, then into the running
+  // value with the reducer operator (the running value wins ties).
+  void fold_launches() const
+  {
+    std::vector<Launch> pending;
+    {
+      std::lock_guard<std::mutex> g(m_state->lock);
+      pending.swap(m_state->launches);
+    }
+    for (const Launch& l : pending)
+    {
+      resources::detail::check_hip(hipStreamSynchronize(l.stream), "reducer wait");
+      T r = m_state->identity;
+      for (int b = 0; b < l.nblocks; ++b) r = Tree{}(r, l.slots[b]);
+      if (reduce::detail::differs(r, m_state->identity))
+      {
+        m_state->value = Op{}(m_state->value, r);
+      }
+      detail::release_slots(l.slots);
+    }
+  }
+
+  State* m_state = nullptr;
+  T* m_slots = nullptr;
+  T m_identity;
+  mutable T m_val;
+  bool m_owner = false;
+  bool m_device_private = false;
+};
+
+namespace detail
+{
+
+// ---- expt parameter reductions --------------------------------------------
+
+template<typename... Ps, size_t... I>
+void setup_params(std::tuple<Ps...>& ps, int nblocks, std::index_sequence<I...>)
+{
+  (void)std::initializer_list<int>{
+      0, ((std::get<I>(ps).device_partials =
+               acquire_slots<typename Ps::value_type>(nblocks),
+           std::get<I>(ps).nblocks = nblocks),
+          0)...};
+}
+
+template<typename P, typename V>
+__device__ inline void finalize_one(P& p, V& v)
+{
+  using T = typename P::value_type;
+  using Tree = typename expt::detail::valop_tree<typename P::valop_type>::type;
+  store_block_value<T, Tree>(v.val, static_cast<T*>(p.device_partials));
+}
+
+template<typename... Ps, typename... Vs, size_t... I>
+__device__ inline void finalize_params(std::tuple<Ps...>& ps, std::tuple<Vs...>& vs,
+                                       std::index_sequence<I...>)
+{
+  (void)std::initializer_list<int>{0, (finalize_one(std::get<I>(ps), std::get<I>(vs)), 0)...};
+}
+
+template<typename P>
+void resolve_one(P& p)
+{
+  using T = typename P::value_type;
+  using Tree = typename expt::detail::valop_tree<typename P::valop_type>::type;
+  const T* slots = static_cast<const T*>(p.device_partials);
+  T r = typename P::valop_type().val;
+  for (int b = 0; b < p.nblocks; ++b) r = Tree{}(r, slots[b]);
+  expt::detail::combine_target(p, r);
+  release_slots(p.device_partials);
+  p.device_partials = nullptr;
+}
+
+template<typename... Ps, size_t... I>
+void resolve_params(std::tuple<Ps...>& ps, std::index_sequence<I...>)
+{
+  (void)std::initializer_list<int>{0, (resolve_one(std::get<I>(ps)), 0)...};
+}
+
+// ---- the forall kernel -------------------------------------------------------
+
+template<int BLOCK, typename Seg, typename Body, typename Reducers>
+__global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, Reducers reducers,
+                                                       Index_type len)
+{
+  Body b(body);  // device-private copy: its reducers fold per block at scope end
+  auto vops = RAJA::detail::make_valops(reducers);
+  const Index_type stride = (Index_type)gridDim.x * BLOCK;
+  for (Index_type k = (Index_type)blockIdx.x * BLOCK + threadIdx.x; k < len; k += stride)
+  {
+    RAJA::detail::invoke_body(b, seg.at(k), vops);
+  }
+  finalize_params(reducers, vops,
+                  std::make_index_sequence<std::tuple_size<Reducers>::value>{});
+}
+
+template<typename Kernel>
+int occupancy_grid(Kernel k, int block)
+{
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||
+      per_cu <= 0)
+  {
+    (void)hipGetLastError();
+    per_cu = 2048 / block;
+  }
+  return per_cu * raja_cdna4_num_cus();
+}
+
+}  // namespace detail
+}  // namespace cdna4
+
+RAJA_CDNA4_DECLARE_REDUCERS(cdna4_reduce, cdna4::Cdna4ReduceImpl)
+
+namespace policy
+{
+namespace cdna4
+{
+
+template<int B, bool Async, int G, typename Iterable, typename Body, typename Pack>
+resources::EventProxy<resources::Cdna4> forall_impl(resources::Cdna4 res,
+                                                    const cdna4_exec_t<B, Async, G>&,
+                                                    Iterable&& it, Body&& body, Pack& pack)
+{
+  using Seg = std::decay_t<Iterable>;
+  using BodyT = std::decay_t<Body>;
+  using Reducers = decltype(pack.reducers);
+  constexpr size_t NP = std::tuple_size<Reducers>::value;
+  const Index_type len = (Index_type)it.size();
+  auto kernel = RAJA::cdna4::detail::forall_kernel<B, Seg, BodyT, Reducers>;
+  static int max_grid = 0;
+  if (max_grid == 0) max_grid = RAJA::cdna4::detail::occupancy_grid(kernel, B);
+  Index_type grid = (len + B - 1) / B;
+  const Index_type cap = (G > 0) ? G : max_grid;
+  if (grid > cap) grid = cap;
+  RAJA::cdna4::detail::setup_params(pack.reducers, (int)grid, std::make_index_sequence<NP>{});
+  hipStream_t stream = res.get_stream();
+  RAJA::cdna4::detail::LaunchInfo& li = RAJA::cdna4::detail::launch_info();
+  li = RAJA::cdna4::detail::LaunchInfo{true, (int)grid, stream};
+  BodyT launch_body(body);  // reducers in the body take their block slots here
+  li.setup = false;
+  Seg seg(it);
+  hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(B), 0, stream, seg, launch_body,
+                     pack.reducers, len);
+  resources::detail::check_hip(hipGetLastError(), "forall launch");
+  if (NP > 0)
+  {
+    res.wait();  // parameter results resolve at the end of forall
+    RAJA::cdna4::detail::resolve_params(pack.reducers, std::make_index_sequence<NP>{});
+  }
+  else if (!Async)
+  {
+    res.wait();
+  }
+  return resources::EventProxy<resources::Cdna4>(res);
+}
+
+}  // namespace cdna4
+}  // namespace policy
+
+}  // namespace RAJA

@@ -1,0 +1,599 @@
+// Front-end test driver: RAJA API calls with seq_exec / omp_parallel_for_exec
+// and, when built by hipcc, the cdna4 policies, checked against each other
+// and against closed forms, following the reference's test cases
+// (test/functional/forall/reduce-basic, test/functional/scan,
+// test/unit/algorithm, test/functional/workgroup, test/unit/resource).
+// Prints one line per check and exits with the number of failures.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "RAJA/RAJA.hpp"
+
+static int g_fail = 0;
+static int g_pass = 0;
+
+#define CHECK(cond, name)                                                     \
+  do                                                                          \
+  {                                                                           \
+    if (cond)                                                                 \
+    {                                                                         \
+      ++g_pass;                                                               \
+    }                                                                         \
+    else                                                                      \
+    {                                                                         \
+      ++g_fail;                                                               \
+      std::printf("FAIL %s (%s:%d)\n", name, __FILE__, __LINE__);             \
+    }                                                                         \
+  } while (0)
+
+// splitmix64 of (seed << 40) + i: the same generator the library uses
+static uint64_t mix(uint64_t seed, uint64_t i)
+{
+  uint64_t z = (seed << 40) + i + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// ---- host policies ------------------------------------------------------------
+
+template<typename EXEC, typename REDUCE>
+void test_host_reducers(const char* tag)
+{
+  const int N = 2057;
+  std::vector<int> a(N);
+  std::vector<double> d(N);
+  for (int i = 0; i < N; ++i)
+  {
+    a[i] = (int)(mix(1, i) % 100);
+    d[i] = (double)a[i];
+  }
+  const int planted = N * 2 / 3;
+  a[planted] = -7;
+  d[planted] = -7.0;
+  long long ref_sum = 0;
+  for (int v : a) ref_sum += v;
+
+  int* ap = a.data();
+  double* dp = d.data();
+  RAJA::ReduceSum<REDUCE, int> sum(0), sum2(2);
+  RAJA::ReduceMin<REDUCE, int> mn(std::numeric_limits<int>::max());
+  RAJA::ReduceMax<REDUCE, double> mx(-1.0);
+  RAJA::ReduceMinLoc<REDUCE, double> mloc(std::numeric_limits<double>::max(), -1);
+  RAJA::ReduceMaxLoc<REDUCE, int> xloc(std::numeric_limits<int>::min(), -1);
+  RAJA::ReduceBitOr<REDUCE, int> bor(0);
+  RAJA::ReduceBitAnd<REDUCE, int> band(~0);
+  RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=](RAJA::Index_type i) {
+    sum += ap[i];
+    sum2 += ap[i];
+    mn.min(ap[i]);
+    mx.max(dp[i]);
+    mloc.minloc(dp[i], i);
+    xloc.maxloc(ap[i], i);
+    bor |= (ap[i] & 0xF);
+    band &= (ap[i] | 0x100);
+  });
+  std::string t(tag);
+  CHECK(sum.get() == ref_sum, (t + " ReduceSum").c_str());
+  CHECK(sum2.get() == ref_sum + 2, (t + " ReduceSum init offset").c_str());
+  CHECK(mn.get() == -7, (t + " ReduceMin").c_str());
+  CHECK(mx.get() == 99.0, (t + " ReduceMax").c_str());
+  CHECK(mloc.get() == -7.0 && mloc.getLoc() == planted, (t + " ReduceMinLoc").c_str());
+  int first99 = -1;
+  for (int i = 0; i < N; ++i)
+    if (a[i] == 99)
+    {
+      first99 = i;
+      break;
+    }
+  CHECK(xloc.get() == 99 && xloc.getLoc() == first99, (t + " ReduceMaxLoc first index").c_str());
+  CHECK(bor.get() == 0xF, (t + " ReduceBitOr").c_str());
+  // accumulation across two foralls, then reset
+  RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=](RAJA::Index_type i) { sum += ap[i]; });
+  CHECK(sum.get() == 2 * ref_sum, (t + " ReduceSum accumulates").c_str());
+  sum.reset(5);
+  RAJA::forall<EXEC>(RAJA::RangeSegment(0, 10), [=](RAJA::Index_type i) { sum += 1; });
+  CHECK(sum.get() == 15, (t + " ReduceSum reset").c_str());
+}
+
+template<typename EXEC>
+void test_host_params(const char* tag)
+{
+  const int N = 1000;
+  std::vector<int> a(N);
+  for (int i = 0; i < N; ++i) a[i] = (i % 2 == 0) ? 1 : -1;
+  a[0] = 3;
+  a[N / 2] = -100;
+  a[N / 2 + 1] = 100;
+  int* ap = a.data();
+  int s = 0, mn = std::numeric_limits<int>::max(), mx = std::numeric_limits<int>::min();
+  RAJA::expt::ValLoc<int> ml(std::numeric_limits<int>::max(), -1);
+  int m2 = std::numeric_limits<int>::max();
+  RAJA::Index_type l2 = -1;
+  using VSUM = RAJA::expt::ValOp<int, RAJA::operators::plus>;
+  using VMIN = RAJA::expt::ValOp<int, RAJA::operators::minimum>;
+  using VMAX = RAJA::expt::ValOp<int, RAJA::operators::maximum>;
+  using VLMIN = RAJA::expt::ValLocOp<int, RAJA::Index_type, RAJA::operators::minimum>;
+  RAJA::forall<EXEC>(
+      RAJA::RangeSegment(0, N), RAJA::expt::Reduce<RAJA::operators::plus>(&s),
+      RAJA::expt::Reduce<RAJA::operators::minimum>(&mn),
+      RAJA::expt::Reduce<RAJA::operators::maximum>(&mx),
+      RAJA::expt::Reduce<RAJA::operators::minimum>(&ml), RAJA::Name("params"),
+      RAJA::expt::ReduceLoc<RAJA::operators::minimum>(&m2, &l2),
+      [=](RAJA::Index_type i, VSUM& vs, VMIN& vmn, VMAX& vmx, VLMIN& vml, VLMIN& vml2) {
+        vs += ap[i];
+        vmn.min(ap[i]);
+        vmx.max(ap[i]);
+        vml.minloc(ap[i], i);
+        vml2.minloc(ap[i], i);
+      });
+  int ref = 0;
+  for (int v : a) ref += v;
+  std::string t(tag);
+  CHECK(s == ref, (t + " expt sum").c_str());
+  CHECK(mn == -100 && mx == 100, (t + " expt min/max").c_str());
+  CHECK(ml.getVal() == -100 && ml.getLoc() == N / 2, (t + " expt ValLoc").c_str());
+  CHECK(m2 == -100 && l2 == N / 2, (t + " expt ReduceLoc").c_str());
+}
+
+template<typename EXEC>
+void test_host_scans_sorts(const char* tag)
+{
+  std::string t(tag);
+  for (int n : {0, 357, 32000})
+  {
+    std::vector<long> in(n), out(n);
+    std::iota(in.begin(), in.end(), 1);
+    RAJA::inclusive_scan<EXEC>(RAJA::make_span(in.data(), n), RAJA::make_span(out.data(), n));
+    bool ok = true;
+    for (long i = 0; i < n; ++i) ok = ok && out[i] == (i + 1) * (i + 2) / 2;
+    CHECK(ok, (t + " inclusive_scan iota").c_str());
+    RAJA::exclusive_scan<EXEC>(RAJA::make_span(in.data(), n), RAJA::make_span(out.data(), n),
+                               RAJA::operators::plus<long>{}, 13L);
+    ok = true;
+    for (long i = 0; i < n; ++i) ok = ok && out[i] == 13 + i * (i + 1) / 2;
+    CHECK(ok, (t + " exclusive_scan iota offset 13").c_str());
+    std::vector<long> ip(in);
+    RAJA::exclusive_scan_inplace<EXEC>(RAJA::make_span(ip.data(), n), RAJA::operators::maximum<long>{});
+    ok = true;
+    for (long i = 1; i < n; ++i) ok = ok && ip[i] == i;
+    CHECK(ok, (t + " exclusive_scan_inplace max").c_str());
+  }
+  const int n = 10000;
+  std::vector<int> k(n), v(n);
+  for (int i = 0; i < n; ++i)
+  {
+    k[i] = (int)(mix(3, i) % (2 * n + 1)) - n;
+    v[i] = i;
+  }
+  std::vector<int> ks(k), vs(v);
+  RAJA::stable_sort_pairs<EXEC>(RAJA::make_span(ks.data(), n), RAJA::make_span(vs.data(), n));
+  bool ok = true;
+  for (int i = 1; i < n; ++i)
+    ok = ok && (ks[i - 1] < ks[i] || (ks[i - 1] == ks[i] && vs[i - 1] < vs[i]));
+  for (int i = 0; i < n; ++i) ok = ok && k[vs[i]] == ks[i];
+  CHECK(ok, (t + " stable_sort_pairs").c_str());
+  std::vector<int> kd(k);
+  RAJA::sort<EXEC>(RAJA::make_span(kd.data(), n), RAJA::operators::greater<int>{});
+  ok = true;
+  for (int i = 1; i < n; ++i) ok = ok && kd[i - 1] >= kd[i];
+  CHECK(ok, (t + " sort greater").c_str());
+}
+
+template<typename WORK>
+void test_host_workgroup(const char* tag)
+{
+  using Pol = RAJA::WorkGroupPolicy<WORK, RAJA::ordered, RAJA::constant_stride_array_of_objects>;
+  using Pool = RAJA::WorkPool<Pol, RAJA::Index_type, RAJA::xargs<int>, std::allocator<char>>;
+  const int N = 4096;
+  std::vector<int> a(N, -1);
+  int* ap = a.data();
+  Pool pool;
+  std::vector<std::pair<int, int>> ranges = {{0, 100}, {100, 100}, {150, 1024}, {2000, 4096}};
+  for (auto r : ranges)
+  {
+    pool.enqueue(RAJA::RangeSegment(r.first, r.second), [=](RAJA::Index_type i, int off) { ap[i] = (int)i + off; });
+  }
+  auto group = pool.instantiate();
+  CHECK(pool.num_loops() == 0 && group.num_loops() == 3, (std::string(tag) + " workgroup enqueue/instantiate").c_str());
+  group.run(5);
+  bool ok = true;
+  for (int i = 0; i < N; ++i)
+  {
+    const bool inside = (i < 100) || (i >= 150 && i < 1024) || (i >= 2000);
+    ok = ok && (inside ? a[i] == i + 5 : a[i] == -1);
+  }
+  CHECK(ok, (std::string(tag) + " workgroup i+5").c_str());
+}
+
+// ---- cdna4 --------------------------------------------------------------------------
+
+#if defined(RAJA_ENABLE_CDNA4)
+
+template<typename T>
+T* to_device(const std::vector<T>& h)
+{
+  RAJA::resources::Cdna4 r = RAJA::resources::Cdna4::get_default();
+  T* d = r.allocate<T>(h.size() + 1);
+  r.memcpy(d, h.data(), sizeof(T) * h.size());
+  r.wait();
+  return d;
+}
+
+template<typename T>
+std::vector<T> to_host(const T* d, size_t n)
+{
+  std::vector<T> h(n);
+  RAJA::resources::Cdna4 r = RAJA::resources::Cdna4::get_default();
+  r.memcpy(h.data(), d, sizeof(T) * n);
+  r.wait();
+  return h;
+}
+
+template<typename T>
+void dfree(T* p)
+{
+  RAJA::resources::Cdna4::get_default().deallocate(p);
+}
+
+void test_cdna4_forall_reducers()
+{
+  using EXEC = RAJA::cdna4_exec<256>;
+  using RED = RAJA::cdna4_reduce;
+  for (int N : {1, 28, 642, 2057, 100003, 1 << 20})
+  {
+    std::vector<int> a(N);
+    std::vector<double> d(N);
+    for (int i = 0; i < N; ++i)
+    {
+      a[i] = (int)(mix(1, i) % 100);
+      d[i] = (double)((int)(mix(2, i) % 5));  // heavy ties
+    }
+    const int planted = N * 2 / 3;
+    a[planted] = -7;
+    long long ref = 0;
+    for (int v : a) ref += v;
+    int first_min_d = (int)(std::min_element(d.begin(), d.end()) - d.begin());
+    int* ap = to_device(a);
+    double* dp = to_device(d);
+    RAJA::ReduceSum<RED, long long> sum(0), sum2(2);
+    RAJA::ReduceMin<RED, int> mn;
+    RAJA::ReduceMax<RED, int> mx;
+    RAJA::ReduceMinLoc<RED, double> ml;
+    RAJA::ReduceMaxLoc<RED, int> xl;
+    RAJA::ReduceBitOr<RED, int> bo(0);
+    RAJA::ReduceBitAnd<RED, int> ba(~0);
+    RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+      sum += ap[i];
+      sum2 += ap[i];
+      mn.min(ap[i]);
+      mx.max(ap[i]);
+      ml.minloc(dp[i], i);
+      xl.maxloc(ap[i], i);
+      bo |= (ap[i] & 0xF);
+      ba &= (ap[i] | 0x100);
+    });
+    int first_max_a = (int)(std::max_element(a.begin(), a.end()) - a.begin());
+    int ref_and = ~0, ref_or = 0;
+    for (int v : a)
+    {
+      ref_and &= (v | 0x100);
+      ref_or |= (v & 0xF);
+    }
+    CHECK(sum.get() == ref, "cdna4 ReduceSum");
+    CHECK(sum2.get() == ref + 2, "cdna4 ReduceSum init offset");
+    CHECK(mn.get() == *std::min_element(a.begin(), a.end()), "cdna4 ReduceMin");
+    CHECK(mx.get() == *std::max_element(a.begin(), a.end()), "cdna4 ReduceMax");
+    CHECK(ml.get() == d[first_min_d] && ml.getLoc() == first_min_d, "cdna4 ReduceMinLoc first index");
+    CHECK(xl.get() == a[first_max_a] && xl.getLoc() == first_max_a, "cdna4 ReduceMaxLoc first index");
+    CHECK(bo.get() == ref_or && ba.get() == ref_and, "cdna4 ReduceBitOr/BitAnd");
+    RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) { sum += ap[i]; });
+    CHECK(sum.get() == 2 * ref, "cdna4 ReduceSum accumulates across foralls");
+    sum.reset(5);
+    RAJA::forall<EXEC>(RAJA::RangeSegment(0, 10), [=] RAJA_DEVICE(RAJA::Index_type) { sum += 1; });
+    CHECK(sum.get() == 15, "cdna4 ReduceSum reset");
+    RAJA::ReduceMinLoc<RED, double> init_dom(-1.0, -1);
+    RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) { init_dom.minloc(dp[i], i); });
+    CHECK(init_dom.get() == -1.0 && init_dom.getLoc() == -1, "cdna4 MinLoc init dominates");
+    dfree(ap);
+    dfree(dp);
+  }
+}
+
+void test_cdna4_daxpy_and_params()
+{
+  const int N = 1 << 20;
+  std::vector<double> x(N), y(N);
+  for (int i = 0; i < N; ++i)
+  {
+    x[i] = (double)(mix(0, i) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+    y[i] = (double)(mix(7, i) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  }
+  double* xp = to_device(x);
+  double* yp = to_device(y);
+  const double a = 2.5;
+  RAJA::forall<RAJA::cdna4_exec<256>>(RAJA::RangeSegment(0, N),
+                                      [=] RAJA_DEVICE(RAJA::Index_type i) { yp[i] = a * xp[i] + yp[i]; });
+  std::vector<double> yr(y);
+  for (int i = 0; i < N; ++i) yr[i] = a * x[i] + y[i];
+  std::vector<double> yh = to_host(yp, N);
+  CHECK(std::memcmp(yh.data(), yr.data(), sizeof(double) * N) == 0, "cdna4 daxpy bit-exact (no contraction)");
+  double dot = 0.0, mn = std::numeric_limits<double>::max();
+  RAJA::expt::ValLoc<double> ml(std::numeric_limits<double>::max(), -1);
+  using VSUM = RAJA::expt::ValOp<double, RAJA::operators::plus>;
+  using VMIN = RAJA::expt::ValOp<double, RAJA::operators::minimum>;
+  using VLMIN = RAJA::expt::ValLocOp<double, RAJA::Index_type, RAJA::operators::minimum>;
+  RAJA::forall<RAJA::cdna4_exec<256>>(
+      RAJA::RangeSegment(0, N), RAJA::expt::Reduce<RAJA::operators::plus>(&dot),
+      RAJA::expt::Reduce<RAJA::operators::minimum>(&mn),
+      RAJA::expt::Reduce<RAJA::operators::minimum>(&ml),
+      [=] RAJA_DEVICE(RAJA::Index_type i, VSUM& s, VMIN& m, VLMIN& l) {
+        s += xp[i] * yp[i];
+        m.min(xp[i]);
+        l.minloc(xp[i], i);
+      });
+  double ref = 0.0, absum = 0.0;
+  for (int i = 0; i < N; ++i)
+  {
+    ref += x[i] * yr[i];
+    absum += std::fabs(x[i] * yr[i]);
+  }
+  const int im = (int)(std::min_element(x.begin(), x.end()) - x.begin());
+  CHECK(std::fabs(dot - ref) <= 2 * 20 * 2.2e-16 * absum * 2, "cdna4 expt dot within bound");
+  CHECK(mn == x[im], "cdna4 expt min");
+  CHECK(ml.getVal() == x[im] && ml.getLoc() == im, "cdna4 expt ValLoc");
+  // repeatability: same launch shape, same bits
+  double dot2 = 0.0;
+  RAJA::forall<RAJA::cdna4_exec<256>>(RAJA::RangeSegment(0, N), RAJA::expt::Reduce<RAJA::operators::plus>(&dot2),
+                                      [=] RAJA_DEVICE(RAJA::Index_type i, VSUM& s) { s += xp[i] * yp[i]; });
+  CHECK(dot2 == dot, "cdna4 expt sum deterministic");
+  dfree(xp);
+  dfree(yp);
+}
+
+void test_cdna4_segments_resources()
+{
+  const int N = 5000;
+  std::vector<int> idx;
+  for (int i = 0; i < N; i += 3) idx.push_back(N - 1 - i);
+  RAJA::resources::Cdna4 res = RAJA::resources::Cdna4::get_default();
+  RAJA::TypedListSegment<int> list(idx, res);
+  std::vector<int> a(N, 0);
+  int* ap = to_device(a);
+  RAJA::ReduceSum<RAJA::cdna4_reduce, long> cnt(0);
+  RAJA::forall<RAJA::cdna4_exec<128>>(list, [=] RAJA_DEVICE(int i) {
+    ap[i] = 1;
+    cnt += 1;
+  });
+  std::vector<int> h = to_host(ap, N);
+  bool ok = true;
+  for (int i = 0; i < N; ++i) ok = ok && h[i] == (((N - 1 - i) % 3 == 0) ? 1 : 0);
+  CHECK(ok && cnt.get() == (long)idx.size(), "cdna4 ListSegment forall");
+  RAJA::forall<RAJA::cdna4_exec<64>>(RAJA::RangeStrideSegment(3, 1029, 3), [=] RAJA_DEVICE(RAJA::Index_type i) { ap[i] = 7; });
+  h = to_host(ap, N);
+  ok = true;
+  for (int i = 3; i < 1029; i += 3) ok = ok && h[i] == 7;
+  CHECK(ok, "cdna4 RangeStrideSegment forall");
+  // async resources with an event dependency (test-resource-Depends)
+  RAJA::resources::Cdna4 r1, r2;
+  std::vector<int> z(N, 0);
+  int* d1 = to_device(z);
+  RAJA::resources::Event e =
+      RAJA::forall<RAJA::cdna4_exec_async<256>>(r1, RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) { d1[i] = (int)i * -1; });
+  r2.wait_for(&e);
+  RAJA::forall<RAJA::cdna4_exec_async<256>>(r2, RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) { d1[i] *= 2; });
+  r2.wait();
+  h = to_host(d1, N);
+  ok = true;
+  for (int i = 0; i < N; ++i) ok = ok && h[i] == -2 * i;
+  CHECK(ok, "cdna4 async resources + event dependency");
+  // empty range: no launch, reducer untouched
+  RAJA::ReduceSum<RAJA::cdna4_reduce, int> e0(3);
+  RAJA::forall<RAJA::cdna4_exec<256>>(RAJA::RangeSegment(5, 5), [=] RAJA_DEVICE(RAJA::Index_type) { e0 += 1; });
+  CHECK(e0.get() == 3, "cdna4 empty range");
+  RAJA::synchronize<RAJA::cdna4_synchronize>();
+  dfree(ap);
+  dfree(d1);
+}
+
+void test_cdna4_scans_sorts()
+{
+  using EXEC = RAJA::cdna4_exec<256>;
+  for (int n : {0, 357, 32000, 1000003})
+  {
+    std::vector<int> in(n), out(n);
+    for (int i = 0; i < n; ++i) in[i] = (int)(mix(3, i) % 16);
+    int* din = to_device(in);
+    int* dout = to_device(out);
+    RAJA::exclusive_scan<EXEC>(RAJA::make_span(din, n), RAJA::make_span(dout, n));
+    std::vector<int> ref(n);
+    RAJA::exclusive_scan<RAJA::seq_exec>(RAJA::make_span(in.data(), n), RAJA::make_span(ref.data(), n));
+    CHECK(to_host(dout, n) == ref, "cdna4 exclusive_scan == seq");
+    RAJA::inclusive_scan<EXEC>(RAJA::make_span(din, n), RAJA::make_span(dout, n), RAJA::operators::maximum<int>{});
+    RAJA::inclusive_scan<RAJA::seq_exec>(RAJA::make_span(in.data(), n), RAJA::make_span(ref.data(), n), RAJA::operators::maximum<int>{});
+    CHECK(to_host(dout, n) == ref, "cdna4 inclusive_scan max == seq");
+    RAJA::exclusive_scan_inplace<EXEC>(RAJA::make_span(din, n), RAJA::operators::plus<int>{}, 13);
+    RAJA::exclusive_scan_inplace<RAJA::seq_exec>(RAJA::make_span(in.data(), n), RAJA::operators::plus<int>{}, 13);
+    CHECK(to_host(din, n) == in, "cdna4 exclusive_scan_inplace offset == seq");
+    dfree(din);
+    dfree(dout);
+  }
+  for (int n : {0, 1, 10, 100, 1000, 10000, 300007})
+  {
+    std::vector<int> k(n), v(n);
+    for (int i = 0; i < n; ++i)
+    {
+      k[i] = (int)(mix(4, i) % (2 * n + 1)) - n;
+      v[i] = i;
+    }
+    int* dk = to_device(k);
+    int* dv = to_device(v);
+    RAJA::sort_pairs<EXEC>(RAJA::make_span(dk, n), RAJA::make_span(dv, n), RAJA::operators::greater<int>{});
+    std::vector<int> rk(k), rv(v);
+    RAJA::stable_sort_pairs<RAJA::seq_exec>(RAJA::make_span(rk.data(), n), RAJA::make_span(rv.data(), n),
+                                            RAJA::operators::greater<int>{});
+    CHECK(to_host(dk, n) == rk && to_host(dv, n) == rv, "cdna4 sort_pairs greater == seq stable");
+    std::vector<double> kd(n);
+    for (int i = 0; i < n; ++i) kd[i] = 0.5 * k[i];
+    double* dkd = to_device(kd);
+    RAJA::stable_sort<EXEC>(RAJA::make_span(dkd, n));
+    std::vector<double> rd(kd);
+    RAJA::stable_sort<RAJA::seq_exec>(RAJA::make_span(rd.data(), n));
+    CHECK(to_host(dkd, n) == rd, "cdna4 stable_sort double == seq");
+    dfree(dk);
+    dfree(dv);
+    dfree(dkd);
+  }
+}
+
+template<typename ORDER, typename DISPATCH>
+void test_cdna4_workgroup_case(const char* name)
+{
+  const int N = 50000;
+  std::vector<int> a(N, -1);
+  int* ap = to_device(a);
+  auto body = [=] RAJA_DEVICE(RAJA::Index_type i, int off) { ap[i] = (int)i + off; };
+  using Body = decltype(body);
+  using Pol = RAJA::WorkGroupPolicy<RAJA::cdna4_work<256>, ORDER, RAJA::constant_stride_array_of_objects, DISPATCH>;
+  using Pool = RAJA::WorkPool<Pol, RAJA::Index_type, RAJA::xargs<int>, std::allocator<char>>;
+  (void)sizeof(Body);
+  Pool pool;
+  std::vector<int> inside(N, 0);
+  for (int l = 0; l < 40; ++l)
+  {
+    int b = (int)(mix(9, 2 * l) % N), len = (int)(mix(9, 2 * l + 1) % 1025);
+    int e = std::min(N, b + len);
+    // keep ranges disjoint: skip overlaps
+    bool clash = false;
+    for (int i = b; i < e; ++i) clash = clash || inside[i];
+    if (clash) continue;
+    for (int i = b; i < e; ++i) inside[i] = 1;
+    pool.enqueue(RAJA::RangeSegment(b, e), body);
+  }
+  auto group = pool.instantiate();
+  auto site = group.run(5);
+  RAJA::resources::Cdna4 r = site.get_resource();
+  r.wait();
+  std::vector<int> h = to_host(ap, N);
+  bool ok = true;
+  for (int i = 0; i < N; ++i) ok = ok && (inside[i] ? h[i] == i + 5 : h[i] == -1);
+  CHECK(ok, name);
+  // run again (tables reused)
+  group.run(7);
+  h = to_host(ap, N);
+  ok = true;
+  for (int i = 0; i < N; ++i) ok = ok && (inside[i] ? h[i] == i + 7 : h[i] == -1);
+  CHECK(ok, name);
+  dfree(ap);
+}
+
+struct PackBody
+{
+  double* buf;
+  const double* var;
+  const int* list;
+  RAJA_HOST_DEVICE void operator()(RAJA::Index_type i) const { buf[i] = var[list[i]]; }
+};
+struct UnpackBody
+{
+  const double* buf;
+  double* var;
+  const int* list;
+  RAJA_HOST_DEVICE void operator()(RAJA::Index_type i) const { var[list[i]] = buf[i]; }
+};
+
+void test_cdna4_workgroups()
+{
+  using IFC = RAJA::indirect_function_call_dispatch;
+  test_cdna4_workgroup_case<RAJA::unordered_cdna4_loop_tiles, IFC>("cdna4 workgroup unordered indirect");
+  test_cdna4_workgroup_case<RAJA::ordered, IFC>("cdna4 workgroup ordered indirect");
+  test_cdna4_workgroup_case<RAJA::reverse_ordered, IFC>("cdna4 workgroup reverse indirect");
+  // halo pack/unpack on a small grid with direct dispatch of two body types
+  const int g = 16, h = 2, s = g + 2 * h, size = s * s * s;
+  std::vector<double> var(size);
+  for (int i = 0; i < size; ++i) var[i] = i;
+  std::vector<int> pl, ul;
+  std::vector<int> lens;
+  const int dirs[26][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}, {-1, -1, 0},
+                           {-1, 1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 0, -1}, {-1, 0, 1}, {1, 0, -1}, {1, 0, 1},
+                           {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}, {-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1},
+                           {-1, 1, 1}, {1, -1, -1}, {1, -1, 1}, {1, 1, -1}, {1, 1, 1}};
+  for (int l = 0; l < 26; ++l)
+  {
+    int pr[3][2], ur[3][2];
+    for (int a = 0; a < 3; ++a)
+    {
+      const int d = dirs[l][a];
+      pr[a][0] = d == 0 ? h : (d < 0 ? h : g);
+      pr[a][1] = d == 0 ? g + h : (d < 0 ? 2 * h : g + h);
+      ur[a][0] = d == 0 ? h : (d < 0 ? 0 : g + h);
+      ur[a][1] = d == 0 ? g + h : (d < 0 ? h : g + 2 * h);
+    }
+    int n = 0;
+    for (int k = pr[2][0]; k < pr[2][1]; ++k)
+      for (int j = pr[1][0]; j < pr[1][1]; ++j)
+        for (int i = pr[0][0]; i < pr[0][1]; ++i, ++n) pl.push_back(i + j * s + k * s * s);
+    for (int k = ur[2][0]; k < ur[2][1]; ++k)
+      for (int j = ur[1][0]; j < ur[1][1]; ++j)
+        for (int i = ur[0][0]; i < ur[0][1]; ++i) ul.push_back(i + j * s + k * s * s);
+    lens.push_back(n);
+  }
+  double* dvar = to_device(var);
+  int* dpl = to_device(pl);
+  int* dul = to_device(ul);
+  std::vector<double> zero(pl.size(), 0.0);
+  double* dbuf = to_device(zero);
+  using Disp = RAJA::direct_dispatch<camp::list<RAJA::RangeSegment, PackBody>, camp::list<RAJA::RangeSegment, UnpackBody>>;
+  using Pol = RAJA::WorkGroupPolicy<RAJA::cdna4_work<256>, RAJA::unordered_cdna4_loop_tiles,
+                                    RAJA::constant_stride_array_of_objects, Disp>;
+  using Pool = RAJA::WorkPool<Pol, RAJA::Index_type, RAJA::xargs<>, std::allocator<char>>;
+  Pool pack_pool, unpack_pool;
+  int off = 0;
+  for (int l = 0; l < 26; ++l)
+  {
+    pack_pool.enqueue(RAJA::RangeSegment(0, lens[l]), PackBody{dbuf + off, dvar, dpl + off});
+    unpack_pool.enqueue(RAJA::RangeSegment(0, lens[l]), UnpackBody{dbuf + off, dvar, dul + off});
+    off += lens[l];
+  }
+  auto pg = pack_pool.instantiate();
+  auto ug = unpack_pool.instantiate();
+  pg.run();
+  ug.run();
+  std::vector<double> res = to_host(dvar, size);
+  std::vector<double> ref(var);
+  for (size_t q = 0; q < pl.size(); ++q) ref[ul[q]] = var[pl[q]];
+  CHECK(res == ref, "cdna4 workgroup halo pack/unpack (direct dispatch)");
+  dfree(dvar);
+  dfree(dpl);
+  dfree(dul);
+  dfree(dbuf);
+}
+
+#endif
+
+int main()
+{
+  test_host_reducers<RAJA::seq_exec, RAJA::seq_reduce>("seq");
+  test_host_reducers<RAJA::omp_parallel_for_exec, RAJA::omp_reduce>("omp");
+  test_host_params<RAJA::seq_exec>("seq");
+  test_host_params<RAJA::omp_parallel_for_exec>("omp");
+  test_host_scans_sorts<RAJA::seq_exec>("seq");
+  test_host_scans_sorts<RAJA::omp_parallel_for_exec>("omp");
+  test_host_workgroup<RAJA::seq_work>("seq");
+  test_host_workgroup<RAJA::omp_work>("omp");
+#if defined(RAJA_ENABLE_CDNA4)
+  test_cdna4_forall_reducers();
+  test_cdna4_daxpy_and_params();
+  test_cdna4_segments_resources();
+  test_cdna4_scans_sorts();
+  test_cdna4_workgroups();
+#endif
+  std::printf("%d passed, %d failed\n", g_pass, g_fail);
+  return g_fail;
+}
