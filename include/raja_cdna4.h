@@ -179,6 +179,39 @@ int raja_cdna4_exclusive_scan(int dtype, int op, const void* in, void* out,
 int raja_cdna4_sort_pairs(int key_dtype, void* keys, void* vals, int val_bytes,
                           int64_t n, int descending, void* stream);
 
+/* ---- batched (segmented) work ---------------------------------------------- */
+/* No single reference entry point: these replace one reference algorithm call
+ * per range -- a forall with ReduceSum, an exclusive_scan, a sort_pairs
+ * (include/RAJA/policy/hip/{reduce,scan,sort}.hpp) -- issued for each of many
+ * small uneven ranges, with one or two kernels over all of them.
+ *
+ * A segment plan is built once from host offsets (nseg + 1 non-decreasing
+ * element offsets; segment s is [offsets[s], offsets[s+1])) and holds the
+ * device segment/tile tables; it is reusable across calls and streams and
+ * its calls can be captured in a hipGraph. */
+typedef struct raja_cdna4_segments_s* raja_cdna4_segments;
+
+int raja_cdna4_segments_create(const int64_t* offsets, int64_t nseg, raja_cdna4_segments* out);
+int raja_cdna4_segments_destroy(raja_cdna4_segments plan);
+
+/* out_dev[s] = op(*init, fold(x[segment s])) for every segment, with the same
+ * combine tree as raja_cdna4_reduce on that segment alone (bit-identical
+ * results).  dtype I32, I64, F32 or F64; op PLUS, MIN or MAX. */
+int raja_cdna4_segmented_reduce(raja_cdna4_segments plan, int dtype, int op, const void* x,
+                                const void* init, void* out_dev, void* stream);
+
+/* Exclusive / inclusive scan of every segment independently (each starts
+ * from *init / its first element); in == out allowed. */
+int raja_cdna4_segmented_exclusive_scan(raja_cdna4_segments plan, int dtype, int op,
+                                        const void* in, void* out, const void* init,
+                                        void* stream);
+int raja_cdna4_segmented_inclusive_scan(raja_cdna4_segments plan, int dtype, int op,
+                                        const void* in, void* out, void* stream);
+
+/* Stable sort of every segment's (key, value) pairs independently. */
+int raja_cdna4_segmented_sort_pairs(raja_cdna4_segments plan, int key_dtype, void* keys,
+                                    void* vals, int val_bytes, int descending, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

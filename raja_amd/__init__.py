@@ -73,6 +73,12 @@ def _declare(L):
                                     ctypes.POINTER(dbl), ctypes.POINTER(dbl),
                                     ctypes.POINTER(dbl)],
         "raja_cdna4_timing_names": [ctypes.c_char_p, ctypes.c_size_t],
+        "raja_cdna4_segments_create": [vp, i64, ctypes.POINTER(vp)],
+        "raja_cdna4_segments_destroy": [vp],
+        "raja_cdna4_segmented_reduce": [vp, i32, i32, vp, vp, vp, vp],
+        "raja_cdna4_segmented_exclusive_scan": [vp, i32, i32, vp, vp, vp, vp],
+        "raja_cdna4_segmented_inclusive_scan": [vp, i32, i32, vp, vp, vp],
+        "raja_cdna4_segmented_sort_pairs": [vp, i32, vp, vp, i32, i32, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -268,6 +274,75 @@ def unpack_multi(out):
     loc = h[4:5].view(torch.int64).item()
     return {"sum": h[0].item(), "min": h[1].item(), "max": h[2].item(),
             "minloc_val": h[3].item(), "minloc_loc": loc}
+
+
+# ---- batched (segmented) work ----------------------------------------------------
+
+class Segments:
+    """A segment plan: offsets (nseg + 1 host integers) -> device tile tables."""
+
+    def __init__(self, offsets):
+        import numpy as np
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        if off.ndim != 1 or off.size < 1:
+            raise RajaError("offsets must be a 1-D array of nseg + 1 entries")
+        self.offsets = off
+        self.nseg = off.size - 1
+        self.total = int(off[-1])
+        h = ctypes.c_void_p()
+        _prep()
+        _check(lib().raja_cdna4_segments_create(off.ctypes.data_as(ctypes.c_void_p), self.nseg,
+                                                ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _check(lib().raja_cdna4_segments_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def segmented_reduce(plan, x, op=operators.plus, init=None, out=None):
+    s = _prep(x)
+    if init is None:
+        init = _IDENT[op](x.dtype)
+    hold, p = _host_scalar(init, x.dtype)
+    if out is None:
+        out = torch.empty(plan.nseg, dtype=x.dtype, device=x.device)
+    _check(lib().raja_cdna4_segmented_reduce(plan.handle, _dt(x), op, x.data_ptr(), p,
+                                             out.data_ptr(), s))
+    return out
+
+
+def segmented_exclusive_scan(plan, in_, out, op=operators.plus, init=None):
+    s = _prep(in_, out)
+    if init is None:
+        init = _IDENT[op](in_.dtype)
+    hold, p = _host_scalar(init, in_.dtype)
+    _check(lib().raja_cdna4_segmented_exclusive_scan(plan.handle, _dt(in_), op, in_.data_ptr(),
+                                                     out.data_ptr(), p, s))
+    return out
+
+
+def segmented_inclusive_scan(plan, in_, out, op=operators.plus):
+    s = _prep(in_, out)
+    _check(lib().raja_cdna4_segmented_inclusive_scan(plan.handle, _dt(in_), op, in_.data_ptr(),
+                                                     out.data_ptr(), s))
+    return out
+
+
+def segmented_sort_pairs(plan, keys, vals=None, comparator=operators.less):
+    s = _prep(keys, vals)
+    vb = 0 if vals is None else vals.element_size()
+    _check(lib().raja_cdna4_segmented_sort_pairs(
+        plan.handle, _dt(keys), keys.data_ptr(), None if vals is None else vals.data_ptr(), vb,
+        1 if comparator == operators.greater else 0, s))
+    return keys, vals
 
 
 # ---- synthetic inputs -------------------------------------------------------------

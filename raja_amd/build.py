@@ -67,7 +67,7 @@ def build(verbose=False, jobs=None):
             sys.stderr.write("=== %s\n%s\n" % (src, log))
         raise RuntimeError("HIP compilation failed: %s" % ", ".join(s for s, _ in failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,--no-undefined", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

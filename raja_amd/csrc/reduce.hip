@@ -4,142 +4,23 @@
 // Reference: the HIP reducers fold per-thread values with a shuffle tree,
 // per-block slots and a last-block-done pass, replicated 32 ways and folded
 // again on the host (include/RAJA/policy/hip/reduce.hpp:127-208, 960-1161).
-// Here the span is cut into fixed tiles of TILE elements.  Kernel 1 folds
-// each tile in a fixed order (4 interleaved accumulators per thread, DPP wave
-// tree, waves in order) and stores the tile value; kernel 2, one block, folds
-// the tile values in a fixed order and applies the caller's init.  The
-// combine tree depends only on n -- not on the grid, the timing or whatever
-// else shares the GPU -- and no floating-point atomics are used.  Two
-// kernels instead of a last-block-done pass: on MI355X a release fence at
-// device scope writes back the whole L2 of the block's XCD, and a
-// same-address device atomic per block serialises across the 8 XCDs; the
-// kernel boundary gives the visibility for free.
-#include "ops.hpp"
-#include "runtime.hpp"
+// Here kernel 1 computes the value of every fixed tile of the span and
+// kernel 2 folds the tile values with the canonical 64-ary tree and applies
+// the caller's init (reduce_tree.hpp).  The combine tree depends only on n
+// -- not on the grid, the timing or whatever else shares the GPU -- and no
+// floating-point atomics are used.  Two kernels instead of a
+// last-block-done pass: on MI355X a release fence at device scope writes
+// back the whole L2 of the block's XCD, and a same-address device atomic per
+// block serialises across the 8 XCDs; the kernel boundary gives the
+// visibility for free.
+#include "reduce_tree.hpp"
 
 namespace
 {
 using namespace RAJA::cdna4::dev;
 
-constexpr int kBlock = 256;
-constexpr int kVecPerChunk = 16;  // 16-byte loads in flight per thread
-constexpr int kChunks = 4;        // chunks per tile
-constexpr int kFinalBlock = 1024;
-
-template<typename T>
-struct VecOf
-{
-  static constexpr int n = 16 / sizeof(T);
-  typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
-};
-
-template<typename T>
-constexpr int64_t tile_elems()
-{
-  return (int64_t)kBlock * kVecPerChunk * kChunks * VecOf<T>::n;
-}
-
-// Fold of one tile by the whole block.  Thread t owns vectors k*B + t of the
-// tile (k = 0..63), accumulated into acc[k % 4]; returns the tile value.
-template<typename T, typename Op, bool ALIGNED>
-__device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x, int64_t n,
-                                                        int64_t tile,
-                                                        typename Op::acc_t* s_scratch)
-{
-  using acc_t = typename Op::acc_t;
-  using vec_t = typename VecOf<T>::type;
-  constexpr int VEC = VecOf<T>::n;
-  constexpr int64_t TILE = tile_elems<T>();
-  const Op op{};
-  const int t = (int)threadIdx.x;
-  const int64_t base = tile * TILE;
-  acc_t acc[4] = {Op::ident(), Op::ident(), Op::ident(), Op::ident()};
-  if (ALIGNED && base + TILE <= n)
-  {
-    for (int c = 0; c < kChunks; ++c)
-    {
-      vec_t v[kVecPerChunk];
-#pragma unroll
-      for (int j = 0; j < kVecPerChunk; ++j)
-      {
-        const int k = c * kVecPerChunk + j;
-        v[j] = *reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC);
-      }
-#pragma unroll
-      for (int j = 0; j < kVecPerChunk; ++j)
-      {
-        const int k = c * kVecPerChunk + j;
-        const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
-#pragma unroll
-        for (int e = 0; e < VEC; ++e)
-        {
-          acc[j & 3] = op(acc[j & 3], Op::lift(v[j][e], i0 + e));
-        }
-      }
-    }
-  }
-  else
-  {
-    for (int k = 0; k < kVecPerChunk * kChunks; ++k)
-    {
-      const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e)
-      {
-        if (i0 + e < n)
-        {
-          acc[k & 3] = op(acc[k & 3], Op::lift(x[i0 + e], i0 + e));
-        }
-      }
-    }
-  }
-  acc_t a = op(op(acc[0], acc[1]), op(acc[2], acc[3]));
-  return block_reduce(a, op, s_scratch);
-}
-
-template<typename T, typename Op, bool ALIGNED>
-__global__ __launch_bounds__(kBlock) void reduce_tiles_kernel(
-    const T* __restrict__ x, int64_t n, int64_t ntiles, typename Op::acc_t* __restrict__ partials)
-{
-  using acc_t = typename Op::acc_t;
-  __shared__ acc_t s_scratch[max_waves];
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-  {
-    acc_t v = fold_tile<T, Op, ALIGNED>(x, n, tile, s_scratch);
-    if (threadIdx.x == 0)
-    {
-      partials[tile] = v;
-    }
-  }
-}
-
-// Fixed-order fold of the tile values: thread t takes tiles t, t+1024, ...
-// into 8 interleaved accumulators, then the block tree.
-template<typename Op>
-__global__ __launch_bounds__(kFinalBlock) void reduce_final_kernel(
-    const typename Op::acc_t* __restrict__ partials, int64_t ntiles, typename Op::acc_t init,
-    typename Op::acc_t* __restrict__ out)
-{
-  using acc_t = typename Op::acc_t;
-  const Op op{};
-  __shared__ acc_t s_scratch[max_waves];
-  acc_t acc[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = Op::ident();
-  int j = 0;
-  for (int64_t k = threadIdx.x; k < ntiles; k += kFinalBlock)
-  {
-    acc[j] = op(acc[j], partials[k]);
-    j = (j + 1) & 7;
-  }
-  acc_t a = op(op(op(acc[0], acc[1]), op(acc[2], acc[3])),
-               op(op(acc[4], acc[5]), op(acc[6], acc[7])));
-  a = block_reduce(a, op, s_scratch);
-  if (threadIdx.x == 0)
-  {
-    *out = Op::with_init(init, a);
-  }
-}
+constexpr int kFinalWaves = 16;
+constexpr int kFinalMaxGroups = 1024;  // spans up to 2^26 tiles
 
 template<typename T, typename Op>
 int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
@@ -151,30 +32,28 @@ int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
   if (n > 0 && !x) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "input is NULL");
   rc::StreamCtx* c = nullptr;
   RC_TRY(rc::get_ctx(s, &c));
-  const int64_t ntiles = (n == 0) ? 1 : rc::ceil_div(n, tile_elems<T>());
-  RC_TRY(rc::ensure_tmp(c, (size_t)ntiles * sizeof(acc_t)));
+  const int64_t ntiles = rc::ceil_div(n, rt::tile_elems<T>());
+  if (ntiles > (int64_t)kFinalMaxGroups * 64)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "span too long for one reduction");
+  }
+  RC_TRY(rc::ensure_tmp(c, (size_t)(ntiles > 0 ? ntiles : 1) * sizeof(acc_t)));
   const int64_t max_blocks = (int64_t)rc::num_cus() * 8;
-  const unsigned grid = (unsigned)(ntiles < max_blocks ? ntiles : max_blocks);
-  const bool aligned = (((uintptr_t)x) % 16) == 0;
   acc_t* partials = (acc_t*)c->tmp;
   rc::TimedScope ts(name, s);
-  if (aligned)
+  if (ntiles > 0)
   {
-    hipLaunchKernelGGL((reduce_tiles_kernel<T, Op, true>), dim3(grid), dim3(kBlock), 0, s, x, n,
-                       ntiles, partials);
+    const unsigned grid = (unsigned)(ntiles < max_blocks ? ntiles : max_blocks);
+    hipLaunchKernelGGL((rt::tiles_kernel<T, Op>), dim3(grid), dim3(rt::kBlock), 0, s, x, n, ntiles,
+                       partials);
+    RC_HIP(hipGetLastError());
   }
-  else
-  {
-    hipLaunchKernelGGL((reduce_tiles_kernel<T, Op, false>), dim3(grid), dim3(kBlock), 0, s, x,
-                       n, ntiles, partials);
-  }
-  RC_HIP(hipGetLastError());
-  hipLaunchKernelGGL((reduce_final_kernel<Op>), dim3(1), dim3(kFinalBlock), 0, s, partials, ntiles,
-                     init, out);
+  hipLaunchKernelGGL((rt::final_kernel<Op, kFinalWaves, kFinalMaxGroups>), dim3(1),
+                     dim3(kFinalWaves * 64), 0, s, partials, (const uint32_t*)nullptr,
+                     (const uint32_t*)nullptr, ntiles, init, out);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }
-
 
 template<typename T>
 int reduce_typed(int op, const void* x, int64_t n, const void* init, void* out, hipStream_t s)

@@ -1,0 +1,174 @@
+// The canonical reduction tree shared by the span reductions and the
+// segmented (batched) reductions.
+//
+// A range of n elements is cut into tiles of TILE elements from its start.
+//   tile value : 256 threads; thread t folds vectors t, t+256, ..., (64 of
+//                them, 16-byte each) into 4 interleaved accumulators, the
+//                accumulators pairwise, then the block tree (DPP inside each
+//                wave, the 4 wave values in order);
+//   range value: a 64-ary tree over the tile values -- each level folds
+//                groups of 64 consecutive values (identity-padded) with the
+//                DPP wave tree -- then op(init, value).
+// The tree is a function of n alone: any grid, any number of waves working
+// on the tile values, and any batch the range is part of give the same
+// bits.  A range reduced by the span entry point and the same range as one
+// segment of a batch agree exactly.
+#pragma once
+
+#include "ops.hpp"
+#include "runtime.hpp"
+
+namespace rt
+{
+using namespace RAJA::cdna4::dev;
+
+constexpr int kBlock = 256;
+constexpr int kVecPerChunk = 16;  // 16-byte loads in flight per thread
+constexpr int kChunks = 4;        // chunks per tile
+
+template<typename T>
+struct VecOf
+{
+  static constexpr int n = 16 / sizeof(T);
+  typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+
+template<typename T>
+constexpr int64_t tile_elems()
+{
+  return (int64_t)kBlock * kVecPerChunk * kChunks * VecOf<T>::n;
+}
+
+// Value of tile `tile` of the range x[0..n), in every thread of the block.
+// The element -> (thread, accumulator) assignment does not depend on the
+// alignment of x; only the load width does.
+template<typename T, typename Op>
+__device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x, int64_t n,
+                                                        int64_t tile,
+                                                        typename Op::acc_t* s_scratch)
+{
+  using acc_t = typename Op::acc_t;
+  using vec_t = typename VecOf<T>::type;
+  constexpr int VEC = VecOf<T>::n;
+  constexpr int64_t TILE = tile_elems<T>();
+  const Op op{};
+  const int t = (int)threadIdx.x;
+  const int64_t base = tile * TILE;
+  acc_t acc[4] = {Op::ident(), Op::ident(), Op::ident(), Op::ident()};
+  const bool aligned = (((uintptr_t)(x + base)) % 16) == 0;
+  if (aligned && base + TILE <= n)
+  {
+    for (int c = 0; c < kChunks; ++c)
+    {
+      vec_t v[kVecPerChunk];
+#pragma unroll
+      for (int j = 0; j < kVecPerChunk; ++j)
+      {
+        const int k = c * kVecPerChunk + j;
+        v[j] = *reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC);
+      }
+#pragma unroll
+      for (int j = 0; j < kVecPerChunk; ++j)
+      {
+        const int k = c * kVecPerChunk + j;
+        const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+        {
+          acc[j & 3] = op(acc[j & 3], Op::lift(v[j][e], i0 + e));
+        }
+      }
+    }
+  }
+  else
+  {
+    for (int k = 0; k < kVecPerChunk * kChunks; ++k)
+    {
+      const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+      {
+        if (i0 + e < n)
+        {
+          acc[k & 3] = op(acc[k & 3], Op::lift(x[i0 + e], i0 + e));
+        }
+      }
+    }
+  }
+  acc_t a = op(op(acc[0], acc[1]), op(acc[2], acc[3]));
+  return block_reduce(a, op, s_scratch);
+}
+
+// 64-ary tree over `count` tile values, evaluated by the block's waves
+// (blockDim.x = 64 * waves).  s_a holds >= ceil(count/64) values, s_b >=
+// ceil(count/4096) values.  Returns the range value in every thread.
+template<typename Op>
+__device__ __forceinline__ typename Op::acc_t tree_fold(const typename Op::acc_t* vals,
+                                                        int64_t count,
+                                                        typename Op::acc_t* s_a,
+                                                        typename Op::acc_t* s_b)
+{
+  using acc_t = typename Op::acc_t;
+  const Op op{};
+  const int lane = lane_id();
+  const int wave = wave_id();
+  const int nw = (int)(blockDim.x / wave_size);
+  if (count <= 0) return Op::ident();
+  const acc_t* in = vals;
+  acc_t* outs[2] = {s_a, s_b};
+  int level = 0;
+  while (count > 1)
+  {
+    const int64_t groups = (count + wave_size - 1) / wave_size;
+    acc_t* out = outs[level & 1];
+    for (int64_t g = wave; g < groups; g += nw)
+    {
+      const int64_t i = g * wave_size + lane;
+      const acc_t v = (i < count) ? in[i] : Op::ident();
+      const acc_t r = wave_reduce(v, op);
+      if (lane == 0) out[g] = r;
+    }
+    __syncthreads();
+    in = out;
+    count = groups;
+    ++level;
+  }
+  const acc_t r = in[0];
+  __syncthreads();
+  return r;
+}
+
+// Kernel 1 of a span reduction: tile values of x[0..n), grid-stride.
+template<typename T, typename Op>
+__global__ __launch_bounds__(kBlock) void tiles_kernel(const T* __restrict__ x, int64_t n,
+                                                       int64_t ntiles,
+                                                       typename Op::acc_t* __restrict__ partials)
+{
+  using acc_t = typename Op::acc_t;
+  __shared__ acc_t s_scratch[max_waves];
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+  {
+    const acc_t v = fold_tile<T, Op>(x, n, tile, s_scratch);
+    if (threadIdx.x == 0) partials[tile] = v;
+  }
+}
+
+// Kernel 2: the 64-ary tree over the tile values of each segment (one
+// block per segment; a span reduction is one segment) and the init.
+template<typename Op, int WAVES, int MAX_GROUPS>
+__global__ __launch_bounds__(WAVES * 64) void final_kernel(
+    const typename Op::acc_t* __restrict__ partials, const uint32_t* __restrict__ seg_first,
+    const uint32_t* __restrict__ seg_count, int64_t single_count, typename Op::acc_t init,
+    typename Op::acc_t* __restrict__ out)
+{
+  using acc_t = typename Op::acc_t;
+  __shared__ acc_t s_a[MAX_GROUPS];
+  __shared__ acc_t s_b[(MAX_GROUPS + 63) / 64 + 1];
+  const int64_t seg = blockIdx.x;
+  const int64_t first = seg_first ? seg_first[seg] : 0;
+  const int64_t count = seg_count ? seg_count[seg] : single_count;
+  const acc_t r = tree_fold<Op>(partials + first, count, s_a, s_b);
+  if (threadIdx.x == 0) out[seg] = Op::with_init(init, r);
+}
+
+}  // namespace rt

@@ -20,6 +20,7 @@
 //     frozen into the graph) a clear kernel runs first.
 #include "ops.hpp"
 #include "runtime.hpp"
+#include "segments.hpp"
 
 namespace
 {
@@ -82,18 +83,22 @@ __device__ __forceinline__ uint32_t peek(const Desc<T>* d, uint32_t epoch_tag, T
 }
 
 template<typename T, typename Op>
-__device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, uint32_t epoch_tag)
+__device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, int64_t j,
+                                      uint32_t epoch_tag)
 {
+  // Predecessor `back` tiles behind belongs to this range while back <= j
+  // (j = the tile's index inside its range; j = tile for a single span).
   const Op op{};
   const int lane = lane_id();
   T acc = Op::ident();
-  int64_t base = tile - 1;
+  int64_t step = 1;
   while (true)
   {
-    const int64_t t = base - lane;
+    const int64_t back = step + lane;
+    const int64_t t = tile - back;
     T v = Op::ident();
     uint32_t kind = kInc;
-    if (t >= 0)
+    if (back <= j)
     {
       kind = peek<T>(desc + t, epoch_tag, v);
       while (ballot(kind == 0u) != 0ull)
@@ -115,16 +120,19 @@ __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, uint32_
       return acc;
     }
     acc = op(wave_reduce(v, op), acc);
-    base -= wave_size;
+    step += wave_size;
   }
 }
 
 // Tile t is block t.  Workgroups are dispatched in increasing id order on
 // each XCD, so every tile a block waits for is already resident or done.
-template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED>
+// SEG: the tiles of many ranges (segments) come from a tile table; each
+// range starts from `init` and its lookback stops at its own first tile.
+template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED, bool SEG>
 __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64_t n, T init,
                                                       Desc<T>* __restrict__ desc,
-                                                      uint32_t epoch_tag)
+                                                      uint32_t epoch_tag,
+                                                      const rs::TileRec* __restrict__ tiles)
 {
   using S = ScanShape<T>;
   using vec_t = typename S::vec_t;
@@ -135,10 +143,23 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
   __shared__ T s_prefix;
 
   const int64_t tile = blockIdx.x;
+  int64_t j = tile;               // tile index inside its range
+  int64_t tbase = tile * S::TILE;  // first element of the tile
+  int64_t nend = n;               // end of the range
+  if (SEG)
+  {
+    const rs::TileRec r = tiles[tile];
+    j = r.j;
+    tbase = r.seg_start + (int64_t)r.j * S::TILE;
+    nend = r.seg_start + r.seg_len;
+  }
   const int lane = lane_id();
   const int wave = wave_id();
-  const int64_t wbase = tile * S::TILE + (int64_t)wave * S::WCHUNK;
-  const bool full = ALIGNED && (tile + 1) * S::TILE <= n;
+  const int64_t wbase = tbase + (int64_t)wave * S::WCHUNK;
+  const bool aligned = SEG ? ((((uintptr_t)(in + tbase)) | ((uintptr_t)(out + tbase))) % 16) == 0
+                           : ALIGNED;
+  const bool full = aligned && tbase + S::TILE <= nend;
+  n = nend;
 
   T v[V][VEC];
   if (full)
@@ -193,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
   if (wave == 0)
   {
     T prefix;
-    if (tile == 0)
+    if (j == 0)
     {
       prefix = start;
       if (lane == 0) publish<T>(desc + tile, epoch_tag | kInc, op(prefix, tile_agg));
@@ -202,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
     {
       if (lane == 0) publish<T>(desc + tile, epoch_tag | kAgg, tile_agg);
       // Inclusive descriptors already hold `start` (tile 0 folds it in).
-      prefix = lookback<T, Op>(desc, tile, epoch_tag);
+      prefix = lookback<T, Op>(desc, tile, j, epoch_tag);
       if (lane == 0) publish<T>(desc + tile, epoch_tag | kInc, op(prefix, tile_agg));
     }
     if (lane == 0) s_prefix = prefix;
@@ -267,14 +288,18 @@ __global__ void clear_desc_kernel(uint64_t* p, int64_t n)
   }
 }
 
+// table == nullptr: one span of n elements; otherwise the tiles of a
+// segment plan (n = total elements of the plan).
 template<typename T, typename Op, bool EXCLUSIVE>
-int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name)
+int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
+                const rs::TileTable* table = nullptr)
 {
   if (n < 0) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "negative length");
   if (n == 0) return RAJA_CDNA4_SUCCESS;  // empty range: no-op (pattern/scan.hpp:72-75)
   if (!in || !out) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
   using S = ScanShape<T>;
-  const int64_t ntiles = rc::ceil_div(n, S::TILE);
+  const int64_t ntiles = table ? table->ntiles : rc::ceil_div(n, S::TILE);
+  if (ntiles == 0) return RAJA_CDNA4_SUCCESS;
   if (ntiles > 0x7FFFFFFF) return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan too long");
   rc::StreamCtx* c = nullptr;
   RC_TRY(rc::get_ctx(s, &c));
@@ -295,15 +320,23 @@ int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const cha
                        dim3(256), 0, s, (uint64_t*)c->scan_desc, words);
   }
   rc::TimedScope ts(name, s);
-  if (aligned)
+  if (table)
   {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true>), dim3((unsigned)ntiles), dim3(kBlock), 0,
-                       s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag);
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, true>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag,
+                       (const rs::TileRec*)table->d_tiles);
+  }
+  else if (aligned)
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, false>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag,
+                       (const rs::TileRec*)nullptr);
   }
   else
   {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false>), dim3((unsigned)ntiles), dim3(kBlock),
-                       0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag);
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, false>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag,
+                       (const rs::TileRec*)nullptr);
   }
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
@@ -351,6 +384,66 @@ int scan_dispatch(int dtype, int op, const void* in, void* out, int64_t n, const
   }
 }
 
+
+template<typename T, bool EXCLUSIVE>
+int seg_scan_typed(int op, const void* in, void* out, raja_cdna4_segments_s* plan,
+                   const void* init, hipStream_t s)
+{
+  const rs::TileTable* tab = nullptr;
+  RC_TRY(rs::get_table(plan, ScanShape<T>::TILE, &tab));
+  const char* nm = EXCLUSIVE ? "segmented_exclusive_scan" : "segmented_inclusive_scan";
+  switch (op)
+  {
+    case RAJA_CDNA4_OP_PLUS:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpPlus<T>::ident();
+      return launch_scan<T, rk::OpPlus<T>, EXCLUSIVE>((const T*)in, (T*)out, plan->total, iv, s, nm, tab);
+    }
+    case RAJA_CDNA4_OP_MIN:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpMin<T>::ident();
+      return launch_scan<T, rk::OpMin<T>, EXCLUSIVE>((const T*)in, (T*)out, plan->total, iv, s, nm, tab);
+    }
+    case RAJA_CDNA4_OP_MAX:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpMax<T>::ident();
+      return launch_scan<T, rk::OpMax<T>, EXCLUSIVE>((const T*)in, (T*)out, plan->total, iv, s, nm, tab);
+    }
+    default:
+      return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan operator %d not supported", op);
+  }
+}
+
+template<bool EXCLUSIVE>
+int seg_scan_dispatch(int dtype, int op, const void* in, void* out, raja_cdna4_segments_s* plan,
+                      const void* init, hipStream_t s)
+{
+  switch (dtype)
+  {
+    case RAJA_CDNA4_I32: return seg_scan_typed<int32_t, EXCLUSIVE>(op, in, out, plan, init, s);
+    case RAJA_CDNA4_I64: return seg_scan_typed<int64_t, EXCLUSIVE>(op, in, out, plan, init, s);
+    case RAJA_CDNA4_U32: return seg_scan_typed<uint32_t, EXCLUSIVE>(op, in, out, plan, init, s);
+    case RAJA_CDNA4_U64: return seg_scan_typed<uint64_t, EXCLUSIVE>(op, in, out, plan, init, s);
+    case RAJA_CDNA4_F32: return seg_scan_typed<float, EXCLUSIVE>(op, in, out, plan, init, s);
+    case RAJA_CDNA4_F64: return seg_scan_typed<double, EXCLUSIVE>(op, in, out, plan, init, s);
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "dtype %d not supported", dtype);
+  }
+}
+
+}  // namespace
+
+namespace rscan
+{
+int launch_segmented_scan(int dtype, int op, bool exclusive, const void* in, void* out,
+                          raja_cdna4_segments_s* plan, const void* init, hipStream_t s)
+{
+  return exclusive ? seg_scan_dispatch<true>(dtype, op, in, out, plan, init, s)
+                   : seg_scan_dispatch<false>(dtype, op, in, out, plan, init, s);
+}
+}  // namespace rscan
+
+namespace
+{
 }  // namespace
 
 extern "C" {

@@ -1,0 +1,61 @@
+// Segment plans: the device segment/tile table behind the batched entry
+// points.  A plan is built once from host offsets and reused by every
+// segmented reduce/scan/sort call on it (and by graph replays of them).
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace rs
+{
+
+// One tile of one segment.
+struct TileRec
+{
+  int64_t seg_start;  // element offset of the segment
+  int64_t seg_len;    // segment length
+  uint32_t j;         // tile index inside the segment
+  uint32_t seg;       // segment index
+};
+
+// Tiles of one tile size: tile records plus, per segment, its first tile
+// and tile count.
+struct TileTable
+{
+  int64_t tile = 0;
+  int64_t ntiles = 0;
+  TileRec* d_tiles = nullptr;
+  uint32_t* d_first = nullptr;
+  uint32_t* d_count = nullptr;
+  uint32_t max_count = 0;
+};
+
+}  // namespace rs
+
+struct raja_cdna4_segments_s
+{
+  int device = 0;
+  int64_t nseg = 0;
+  int64_t total = 0;
+  std::vector<int64_t> offsets;  // nseg + 1, host copy
+  int64_t* d_offsets = nullptr;
+  std::map<int64_t, rs::TileTable> tables;
+  std::mutex lock;
+};
+
+namespace rs
+{
+// Table for tile size `tile` (built on first use, synchronously).
+int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out);
+}  // namespace rs
+
+// Launchers implemented in scan.hip for segmented scans (SEG mode of the
+// single-pass scan kernel; lookback stops at each segment's first tile).
+namespace rscan
+{
+int launch_segmented_scan(int dtype, int op, bool exclusive, const void* in, void* out,
+                          raja_cdna4_segments_s* plan, const void* init, hipStream_t s);
+}  // namespace rscan

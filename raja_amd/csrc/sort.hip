@@ -21,6 +21,7 @@
 // Ranking follows input order inside a wave (item j, then lane) and across
 // waves and tiles, so the sort is stable.
 #include "ops.hpp"
+#include "radix.hpp"
 #include "runtime.hpp"
 
 namespace
@@ -29,68 +30,11 @@ using namespace RAJA::cdna4::dev;
 
 constexpr int kBlock = 256;  // one thread per digit in the per-digit phases
 constexpr int kWaves = kBlock / wave_size;
-constexpr int kRadix = 256;
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 constexpr uint32_t kCountMask = 0x3FFFFFFFu;
 // A descriptor row: 256 digit words, the tile's hint word, padding.
 constexpr int kRowStride = kRadix + 8;
 constexpr int64_t kMaxSortN = (int64_t)kCountMask;  // counts live in 30 bits
-
-// Order-preserving map of a key to unsigned bits (operator< order).
-template<typename K>
-struct KeyTraits;
-template<>
-struct KeyTraits<uint32_t>
-{
-  using bits_t = uint32_t;
-  __device__ static bits_t to_bits(uint32_t k) { return k; }
-};
-template<>
-struct KeyTraits<int32_t>
-{
-  using bits_t = uint32_t;
-  __device__ static bits_t to_bits(int32_t k) { return (uint32_t)k ^ 0x80000000u; }
-};
-template<>
-struct KeyTraits<float>
-{
-  using bits_t = uint32_t;
-  __device__ static bits_t to_bits(float k)
-  {
-    const uint32_t u = __float_as_uint(k);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-  }
-};
-template<>
-struct KeyTraits<uint64_t>
-{
-  using bits_t = uint64_t;
-  __device__ static bits_t to_bits(uint64_t k) { return k; }
-};
-template<>
-struct KeyTraits<int64_t>
-{
-  using bits_t = uint64_t;
-  __device__ static bits_t to_bits(int64_t k) { return (uint64_t)k ^ 0x8000000000000000ull; }
-};
-template<>
-struct KeyTraits<double>
-{
-  using bits_t = uint64_t;
-  __device__ static bits_t to_bits(double k)
-  {
-    const uint64_t u = (uint64_t)__double_as_longlong(k);
-    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
-  }
-};
-
-template<typename K, bool DESC>
-__device__ __forceinline__ uint32_t digit_of(K k, int shift)
-{
-  auto b = KeyTraits<K>::to_bits(k);
-  if (DESC) b = ~b;
-  return (uint32_t)(b >> shift) & (kRadix - 1);
-}
 
 __global__ void clear_words_kernel(uint32_t* p, int64_t n)
 {

@@ -1,0 +1,139 @@
+"""GPU parity of the batched (segmented) reduce / scan / sort against the
+oracle, and the batching identity: a segment reduced inside a batch gives
+the same bits as the same range reduced alone."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+EPS = 2.0 ** -53
+
+
+def log_uniform_offsets(nseg, seed, max_log2=16, min_len=1):
+    # L = clamp(floor(2^(max_log2 * u)), min_len, 2^max_log2), u from the counter generator
+    from oracle import oracle as O
+    u = np.array([(O.rng_raw(seed, i) >> 11) * 2.0 ** -53 for i in range(nseg)])
+    lens = np.clip(np.floor(2.0 ** (max_log2 * u)), min_len, 2 ** max_log2).astype(np.int64)
+    return np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+
+
+def ragged_offsets(rng, nseg, maxlen, zero_frac=0.1):
+    lens = rng.integers(0, maxlen + 1, nseg)
+    lens[rng.random(nseg) < zero_frac] = 0
+    return np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+
+
+def test_segmented_reduce_f64_bound_and_identity(raja, orc):
+    rng = np.random.default_rng(1)
+    off = ragged_offsets(rng, 700, 70000)
+    n = int(off[-1])
+    x = torch.empty(n, dtype=torch.float64, device=DEV)
+    raja.fill(raja.FILL_F64_NORMAL, x, seed=5)
+    plan = raja.Segments(off)
+    out = raja.segmented_reduce(plan, x, raja.operators.plus, 0.0).cpu().numpy()
+    h = x.cpu().numpy()
+    kn, ab = orc.seg_kn_sum_f64(h, off)
+    lens = np.diff(off)
+    for s in range(plan.nseg):
+        bound = 2 * max(1, math.ceil(math.log2(max(lens[s], 2)))) * EPS * ab[s]
+        assert abs(out[s] - kn[s]) <= bound, s
+    # same bits alone and in the batch
+    for s in list(range(0, plan.nseg, 37)) + [int(np.argmax(lens))]:
+        alone = raja.reduce_sum(x[off[s]:off[s + 1]], 0.0).item()
+        assert alone == out[s], s
+    out2 = raja.segmented_reduce(plan, x, raja.operators.plus, 0.0).cpu().numpy()
+    assert np.array_equal(out, out2)
+    plan.close()
+
+
+def test_segmented_reduce_minmax_int(raja, orc):
+    rng = np.random.default_rng(2)
+    off = ragged_offsets(rng, 300, 5000)
+    x = torch.empty(int(off[-1]), dtype=torch.int32, device=DEV)
+    raja.fill(raja.FILL_I32, x, seed=6, a=-1000, b=1000)
+    h = x.cpu().numpy()
+    plan = raja.Segments(off)
+    for op, f in ((raja.operators.plus, lambda a, i: orc.seq_sum(a, i)),
+                  (raja.operators.minimum, lambda a, i: orc.seq_min(a, i)),
+                  (raja.operators.maximum, lambda a, i: orc.seq_max(a, i))):
+        init = {raja.operators.plus: 3, raja.operators.minimum: 500, raja.operators.maximum: -500}[op]
+        out = raja.segmented_reduce(plan, x, op, init).cpu().numpy()
+        exp = [f(h[off[s]:off[s + 1]], init) for s in range(plan.nseg)]
+        assert np.array_equal(out, np.array(exp, dtype=np.int32))
+    plan.close()
+
+
+@pytest.mark.parametrize("dtype", [torch.int32, torch.int64])
+def test_segmented_scans_exact(raja, orc, dtype):
+    rng = np.random.default_rng(3)
+    off = ragged_offsets(rng, 500, 30000)
+    n = int(off[-1])
+    x32 = torch.empty(n, dtype=torch.int32, device=DEV)
+    raja.fill(raja.FILL_I32, x32, seed=7, a=-5, b=16)
+    x = x32.to(dtype)
+    h = x.cpu().numpy()
+    plan = raja.Segments(off)
+    out = torch.empty_like(x)
+    raja.segmented_exclusive_scan(plan, x, out, raja.operators.plus, 11)
+    got = out.cpu().numpy()
+    for s in range(plan.nseg):
+        seg = h[off[s]:off[s + 1]]
+        assert np.array_equal(got[off[s]:off[s + 1]], orc.seq_exclusive(seg, orc.PLUS, 11)), s
+    raja.segmented_inclusive_scan(plan, x, out, raja.operators.maximum)
+    got = out.cpu().numpy()
+    for s in range(0, plan.nseg, 3):
+        seg = h[off[s]:off[s + 1]]
+        assert np.array_equal(got[off[s]:off[s + 1]], orc.seq_inclusive(seg, orc.MAX)), s
+    y = x.clone()
+    raja.segmented_exclusive_scan(plan, y, y, raja.operators.plus, 0)  # in place
+    ref = np.concatenate([orc.seq_exclusive(h[off[s]:off[s + 1]], orc.PLUS, 0)
+                          for s in range(plan.nseg)] or [np.zeros(0, h.dtype)])
+    assert np.array_equal(y.cpu().numpy(), ref)
+    plan.close()
+
+
+@pytest.mark.parametrize("desc", [False, True])
+def test_segmented_sort_pairs_exact(raja, orc, desc):
+    rng = np.random.default_rng(4)
+    lens = np.concatenate([rng.integers(0, 9000, 400), [20000, 70001]])
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(off[-1])
+    k = torch.empty(n, dtype=torch.uint32, device=DEV)
+    raja.fill(raja.FILL_U32_BITS, k, seed=8, a=12)  # duplicates
+    v = torch.empty(n, dtype=torch.int32, device=DEV)
+    raja.fill(raja.FILL_I32_IOTA, v, a=0)
+    hk = k.view(torch.int32).cpu().numpy().view(np.uint32)
+    hv = v.cpu().numpy()
+    plan = raja.Segments(off)
+    raja.segmented_sort_pairs(plan, k, v, raja.operators.greater if desc else raja.operators.less)
+    ek, ev = orc.seg_stable_sort_pairs_u32(hk, hv, off, descending=desc)
+    assert np.array_equal(k.view(torch.int32).cpu().numpy().view(np.uint32), ek)
+    assert np.array_equal(v.cpu().numpy(), ev)
+    plan.close()
+
+
+def test_segmented_config_size(raja, orc):
+    """20,000 segments, lengths log-uniform in [1, 2^16]: FP64 sums within the
+    bound and bit-identical to per-segment reductions; int32 scan exact."""
+    off = log_uniform_offsets(20000, 6)
+    n = int(off[-1])
+    x = torch.empty(n, dtype=torch.float64, device=DEV)
+    raja.fill(raja.FILL_F64_NORMAL, x, seed=6)
+    plan = raja.Segments(off)
+    out = raja.segmented_reduce(plan, x).cpu().numpy()
+    kn, ab = orc.seg_kn_sum_f64(x.cpu().numpy(), off)
+    lens = np.diff(off)
+    bound = 2 * np.maximum(1, np.ceil(np.log2(np.maximum(lens, 2)))) * EPS * ab
+    assert np.all(np.abs(out - kn) <= bound)
+    for s in range(0, 20000, 997):
+        assert raja.reduce_sum(x[off[s]:off[s + 1]], 0.0).item() == out[s]
+    xi = torch.empty(n, dtype=torch.int32, device=DEV)
+    raja.fill(raja.FILL_I32, xi, seed=3, a=0, b=16)
+    oi = torch.empty_like(xi)
+    raja.segmented_exclusive_scan(plan, xi, oi, raja.operators.plus, 0)
+    assert np.array_equal(oi.cpu().numpy(), orc.seg_exclusive_i32(xi.cpu().numpy(), off, 0))
+    plan.close()
