@@ -50,7 +50,7 @@ int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
   }
   hipLaunchKernelGGL((rt::final_kernel<Op, kFinalWaves, kFinalMaxGroups>), dim3(1),
                      dim3(kFinalWaves * 64), 0, s, partials, (const uint32_t*)nullptr,
-                     (const uint32_t*)nullptr, ntiles, init, out);
+                     (const uint32_t*)nullptr, ntiles, init, out, (const int64_t*)nullptr);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }

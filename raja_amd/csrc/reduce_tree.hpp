@@ -154,12 +154,13 @@ __global__ __launch_bounds__(kBlock) void tiles_kernel(const T* __restrict__ x, 
 }
 
 // Kernel 2: the 64-ary tree over the tile values of each segment (one
-// block per segment; a span reduction is one segment) and the init.
+// block per segment; a span reduction is one segment) and the init;
+// segment s's result goes to out[out_index[s]] (out[s] without an index).
 template<typename Op, int WAVES, int MAX_GROUPS>
 __global__ __launch_bounds__(WAVES * 64) void final_kernel(
     const typename Op::acc_t* __restrict__ partials, const uint32_t* __restrict__ seg_first,
     const uint32_t* __restrict__ seg_count, int64_t single_count, typename Op::acc_t init,
-    typename Op::acc_t* __restrict__ out)
+    typename Op::acc_t* __restrict__ out, const int64_t* __restrict__ out_index)
 {
   using acc_t = typename Op::acc_t;
   __shared__ acc_t s_a[MAX_GROUPS];
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(WAVES * 64) void final_kernel(
   const int64_t first = seg_first ? seg_first[seg] : 0;
   const int64_t count = seg_count ? seg_count[seg] : single_count;
   const acc_t r = tree_fold<Op>(partials + first, count, s_a, s_b);
-  if (threadIdx.x == 0) out[seg] = Op::with_init(init, r);
+  if (threadIdx.x == 0) out[out_index ? out_index[seg] : seg] = Op::with_init(init, r);
 }
 
 }  // namespace rt

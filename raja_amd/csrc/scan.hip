@@ -152,6 +152,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
     j = r.j;
     tbase = r.seg_start + (int64_t)r.j * S::TILE;
     nend = r.seg_start + r.seg_len;
+    out += r.out_delta;  // output element i of the range lands at out[i + out_delta]
   }
   const int lane = lane_id();
   const int wave = wave_id();
@@ -439,6 +440,17 @@ int launch_segmented_scan(int dtype, int op, bool exclusive, const void* in, voi
 {
   return exclusive ? seg_scan_dispatch<true>(dtype, op, in, out, plan, init, s)
                    : seg_scan_dispatch<false>(dtype, op, in, out, plan, init, s);
+}
+
+int launch_table_exclusive_scan_i32(const int32_t* in, int32_t* out, const rs::TileTable& table,
+                                    int64_t total, int32_t init, hipStream_t s)
+{
+  if (table.tile != ScanShape<int32_t>::TILE)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "scan table has the wrong tile size");
+  }
+  return launch_scan<int32_t, rk::OpPlus<int32_t>, true>(in, out, total, init, s,
+                                                         "segmented_exclusive_scan", &table);
 }
 }  // namespace rscan
 

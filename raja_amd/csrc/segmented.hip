@@ -112,7 +112,7 @@ int seg_reduce(raja_cdna4_segments_s* p, const T* x, typename Op::acc_t init,
   }
   hipLaunchKernelGGL((rt::final_kernel<Op, 1, kSegMaxGroups>), dim3((unsigned)p->nseg), dim3(64), 0, s,
                      partials, (const uint32_t*)tab->d_first, (const uint32_t*)tab->d_count,
-                     (int64_t)0, init, out);
+                     (int64_t)0, init, out, (const int64_t*)nullptr);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }
@@ -143,9 +143,9 @@ struct BlockSortShape
 };
 
 template<typename K, typename V, bool DESC, bool HAS_VALS>
-__global__ __launch_bounds__(512) void block_sort_kernel(K* __restrict__ keys, V* __restrict__ vals,
-                                                         const int64_t* __restrict__ offsets,
-                                                         const uint32_t* __restrict__ seg_ids)
+__global__ __launch_bounds__(512) void block_sort_kernel(const K* in_keys, const V* in_vals,
+                                                         K* out_keys, V* out_vals,
+                                                         const rs::SortSeg* __restrict__ segs)
 {
   using S = BlockSortShape<K, V, HAS_VALS>;
   constexpr int BLOCK = S::BLOCK;
@@ -162,17 +162,16 @@ __global__ __launch_bounds__(512) void block_sort_kernel(K* __restrict__ keys, V
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = wave_id();
-  const uint32_t seg = seg_ids[blockIdx.x];
-  const int64_t start = offsets[seg];
-  const int n = (int)(offsets[seg + 1] - start);
+  const rs::SortSeg sg = segs[blockIdx.x];
+  const int n = (int)sg.n;
   K k[ITEMS];
   V v[HAS_VALS ? ITEMS : 1];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j)
   {
     const int i = wave * WCHUNK + j * wave_size + lane;
-    k[j] = (i < n) ? keys[start + i] : K(0);
-    if (HAS_VALS) v[j] = (i < n) ? vals[start + i] : V(0);
+    k[j] = (i < n) ? in_keys[sg.in_start + i] : K(0);
+    if (HAS_VALS) v[j] = (i < n) ? in_vals[sg.in_start + i] : V(0);
   }
   const uint64_t lt_mask = lanemask_lt();
   for (int p = 0; p < P; ++p)
@@ -256,8 +255,8 @@ __global__ __launch_bounds__(512) void block_sort_kernel(K* __restrict__ keys, V
     const int i = wave * WCHUNK + j * wave_size + lane;
     if (i < n)
     {
-      keys[start + i] = k[j];
-      if (HAS_VALS) vals[start + i] = v[j];
+      out_keys[sg.out_keys + i] = k[j];
+      if (HAS_VALS) out_vals[sg.out_vals + i] = v[j];
     }
   }
 }
@@ -267,41 +266,36 @@ int seg_sort(raja_cdna4_segments_s* p, K* keys, V* vals, int key_dtype, int val_
              hipStream_t s)
 {
   using S = BlockSortShape<K, V, HAS_VALS>;
-  std::vector<uint32_t> small;
+  std::vector<rs::SortSeg> small;
   std::vector<int64_t> big;
   for (int64_t g = 0; g < p->nseg; ++g)
   {
-    const int64_t len = p->offsets[(size_t)g + 1] - p->offsets[(size_t)g];
+    const int64_t b = p->offsets[(size_t)g];
+    const int64_t len = p->offsets[(size_t)g + 1] - b;
     if (len <= 1) continue;
-    if (len <= S::TILE) small.push_back((uint32_t)g);
+    if (len <= S::TILE) small.push_back(rs::SortSeg{b, len, b, b});
     else big.push_back(g);
   }
   if (!small.empty())
   {
     rc::StreamCtx* c = nullptr;
     RC_TRY(rc::get_ctx(s, &c));
-    // segment id list: staged through the plan (uploaded once per shape)
-    const rs::TileTable* dummy = nullptr;
-    (void)dummy;
-    uint32_t* d_ids = nullptr;
+    // short-segment records: uploaded once per plan and block shape
+    rs::SortSeg* d_segs = nullptr;
     {
       std::lock_guard<std::mutex> g(p->lock);
-      auto it = p->tables.find(-(int64_t)S::TILE);  // negative key: small-segment id list
-      if (it == p->tables.end())
+      auto it = p->sort_segs.find((int64_t)S::TILE);
+      if (it == p->sort_segs.end())
       {
-        rs::TileTable t;
-        t.tile = -(int64_t)S::TILE;
-        t.ntiles = (int64_t)small.size();
-        RC_HIP(hipMalloc(&t.d_first, sizeof(uint32_t) * small.size()));
-        RC_HIP(hipMemcpy(t.d_first, small.data(), sizeof(uint32_t) * small.size(), hipMemcpyHostToDevice));
-        it = p->tables.emplace(t.tile, t).first;
+        RC_HIP(hipMalloc(&d_segs, sizeof(rs::SortSeg) * small.size()));
+        RC_HIP(hipMemcpy(d_segs, small.data(), sizeof(rs::SortSeg) * small.size(), hipMemcpyHostToDevice));
+        it = p->sort_segs.emplace((int64_t)S::TILE, d_segs).first;
       }
-      d_ids = it->second.d_first;
+      d_segs = it->second;
     }
     rc::TimedScope ts("segmented_block_sort", s);
     hipLaunchKernelGGL((block_sort_kernel<K, V, DESC, HAS_VALS>), dim3((unsigned)small.size()),
-                       dim3(S::BLOCK), 0, s, keys, vals, (const int64_t*)p->d_offsets,
-                       (const uint32_t*)d_ids);
+                       dim3(S::BLOCK), 0, s, keys, vals, keys, vals, (const rs::SortSeg*)d_segs);
     RC_HIP(hipGetLastError());
   }
   for (int64_t g : big)
@@ -336,6 +330,51 @@ int seg_sort_key(raja_cdna4_segments_s* p, int kd, void* keys, void* vals, int v
 }
 
 }  // namespace
+
+namespace rseg
+{
+
+int launch_table_sum_f64(const double* x, const rs::TileTable& table, int64_t nseg, double* out,
+                         const int64_t* d_out_index, hipStream_t s)
+{
+  using Op = rk::OpPlus<double>;
+  if (table.tile != rt::tile_elems<double>() || table.max_count > (uint32_t)kSegMaxGroups * 64)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad table for the batched sum");
+  }
+  if (nseg == 0) return RAJA_CDNA4_SUCCESS;
+  rc::StreamCtx* c = nullptr;
+  RC_TRY(rc::get_ctx(s, &c));
+  RC_TRY(rc::ensure_tmp(c, (size_t)(table.ntiles + 1) * sizeof(double)));
+  double* partials = (double*)c->tmp;
+  rc::TimedScope ts("segmented_reduce", s);
+  if (table.ntiles > 0)
+  {
+    hipLaunchKernelGGL((seg_tiles_kernel<double, Op>), dim3((unsigned)table.ntiles), dim3(rt::kBlock),
+                       0, s, x, (const rs::TileRec*)table.d_tiles, partials);
+    RC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL((rt::final_kernel<Op, 1, kSegMaxGroups>), dim3((unsigned)nseg), dim3(64), 0, s,
+                     partials, (const uint32_t*)table.d_first, (const uint32_t*)table.d_count,
+                     (int64_t)0, 0.0, out, d_out_index);
+  RC_HIP(hipGetLastError());
+  return RAJA_CDNA4_SUCCESS;
+}
+
+int launch_block_sort_u32_i32(const uint32_t* in_keys, const int32_t* in_vals, uint32_t* out_keys,
+                              int32_t* out_vals, const rs::SortSeg* d_segs, int64_t nsmall,
+                              hipStream_t s)
+{
+  if (nsmall == 0) return RAJA_CDNA4_SUCCESS;
+  rc::TimedScope ts("segmented_block_sort", s);
+  hipLaunchKernelGGL((block_sort_kernel<uint32_t, uint32_t, false, true>), dim3((unsigned)nsmall),
+                     dim3(512), 0, s, in_keys, (const uint32_t*)in_vals, out_keys,
+                     (uint32_t*)out_vals, d_segs);
+  RC_HIP(hipGetLastError());
+  return RAJA_CDNA4_SUCCESS;
+}
+
+}  // namespace rseg
 
 extern "C" {
 
@@ -380,6 +419,7 @@ int raja_cdna4_segments_destroy(raja_cdna4_segments_s* p)
     if (kv.second.d_first) RC_HIP(hipFree(kv.second.d_first));
     if (kv.second.d_count) RC_HIP(hipFree(kv.second.d_count));
   }
+  for (auto& kv : p->sort_segs) RC_HIP(hipFree(kv.second));
   if (p->d_offsets) RC_HIP(hipFree(p->d_offsets));
   delete p;
   return RAJA_CDNA4_SUCCESS;

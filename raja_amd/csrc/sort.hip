@@ -23,6 +23,7 @@
 #include "ops.hpp"
 #include "radix.hpp"
 #include "runtime.hpp"
+#include "segments.hpp"
 
 namespace
 {
@@ -363,8 +364,10 @@ struct SortShape
   static constexpr int64_t TILE = (int64_t)BLOCK * ITEMS;
 };
 
+// allow_host_sync: read the histograms back (one stream synchronisation)
+// to skip passes whose digit is the same for every key.
 template<typename K, typename V, bool HAS_VALS, bool DESC>
-int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
+int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s, bool allow_host_sync = true)
 {
   using S = SortShape<K, V, HAS_VALS>;
   constexpr int P = (int)sizeof(K);
@@ -400,7 +403,7 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
   }
   bool active[P];
   for (int p = 0; p < P; ++p) active[p] = true;
-  if (n >= (int64_t)1 << 18)
+  if (allow_host_sync && n >= (int64_t)1 << 18)
   {
     uint32_t* h = (uint32_t*)c->pinned;
     RC_HIP(hipMemcpyAsync(h, hist, hist_bytes, hipMemcpyDeviceToHost, s));
@@ -463,6 +466,15 @@ int sort_key_type(void* keys, void* vals, int val_bytes, int64_t n, int descendi
 }
 
 }  // namespace
+
+namespace rsort
+{
+int sort_pairs_u32_i32_async(uint32_t* keys, int32_t* vals, int64_t n, hipStream_t s)
+{
+  if (n <= 1) return RAJA_CDNA4_SUCCESS;
+  return sort_impl<uint32_t, uint32_t, true, false>(keys, (uint32_t*)vals, n, s, false);
+}
+}  // namespace rsort
 
 extern "C" {
 
