@@ -212,6 +212,91 @@ int raja_cdna4_segmented_inclusive_scan(raja_cdna4_segments plan, int dtype, int
 int raja_cdna4_segmented_sort_pairs(raja_cdna4_segments plan, int key_dtype, void* keys,
                                     void* vals, int val_bytes, int descending, void* stream);
 
+/* ---- request serving ------------------------------------------------------
+ *
+ * A server runs a stream of independent requests (sum, exclusive scan,
+ * sort_pairs), each on caller-owned device inputs and outputs, with every
+ * byte of working memory carved from one device arena sized at creation.
+ * Requests are admitted in arrival order while their working memory fits in
+ * the arena and fewer than max_inflight launch units are running; a request
+ * whose working memory exceeds the whole arena is rejected with
+ * RAJA_CDNA4_ERROR_ADMISSION.  In batched mode, small requests that are
+ * waiting when a unit slot frees up are coalesced into one batch (one table
+ * upload and one or two kernels per request kind); large ones run alone;
+ * units go round-robin over the server's streams.  In per-request mode each
+ * request is its own unit on one stream: the reference's one algorithm call
+ * per request (a RAJA::forall with ReduceSum, RAJA::exclusive_scan or
+ * RAJA::stable_sort_pairs, include/RAJA/pattern/forall.hpp:545-627,
+ * include/RAJA/pattern/scan.hpp:216-295, include/RAJA/pattern/sort.hpp:219-301,
+ * each allocating its temporaries from a pool, include/RAJA/util/basic_mempool.hpp:54-454).
+ * The RAJA counterpart of coalescing is WorkGroup (many small loops in one
+ * launch, include/RAJA/pattern/WorkGroup.hpp:117-530).
+ *
+ * Every request's result is bit-identical whether it runs alone, in a batch
+ * or in either mode (sums use the canonical reduction tree; scans and sorts
+ * are exact).  Request inputs must not alias any output of a request that
+ * may be in flight at the same time. */
+
+enum raja_cdna4_request_kind
+{
+  RAJA_CDNA4_REQ_SUM_F64 = 0,           /* out[0] = sum of in[0..n), FP64          */
+  RAJA_CDNA4_REQ_EXCLUSIVE_SCAN_I32 = 1, /* out = exclusive plus-scan of in, init 0 */
+  RAJA_CDNA4_REQ_SORT_PAIRS_U32_I32 = 2  /* stable ascending sort of (in, in_vals)
+                                            into (out, out_vals)                     */
+};
+
+enum raja_cdna4_serve_mode
+{
+  RAJA_CDNA4_SERVE_BATCHED = 0,
+  RAJA_CDNA4_SERVE_PER_REQUEST = 1
+};
+
+typedef struct
+{
+  int mode;            /* raja_cdna4_serve_mode                                     */
+  int streams;         /* streams for launch units, 1..16 (per-request mode uses 1)  */
+  int max_inflight;    /* launch units in flight, 1..64                              */
+  int max_batch;       /* requests per batch, >= 1                                   */
+  int64_t batch_limit; /* sum/scan requests with n <= batch_limit join batches;
+                          sorts join when n <= 8192                                  */
+  int64_t arena_bytes; /* device working memory of the server                       */
+  int honor_arrivals;  /* 1: a request starts no earlier than its arrival_s;
+                          0: every request has arrived at time 0 (saturation)        */
+  int pad;
+} raja_cdna4_serve_config;
+
+typedef struct
+{
+  int kind;           /* raja_cdna4_request_kind                                     */
+  int pad;
+  int64_t n;          /* elements (pairs for sorts)                                  */
+  const void* in;     /* FP64 / int32 / uint32 keys                                  */
+  const void* in_vals;/* int32 values (sorts)                                        */
+  void* out;          /* 1 double / n int32 / n uint32 keys                          */
+  void* out_vals;     /* n int32 values (sorts)                                      */
+  double arrival_s;   /* seconds after the run starts; non-decreasing                */
+} raja_cdna4_request;
+
+typedef struct
+{
+  int status;         /* RAJA_CDNA4_SUCCESS or RAJA_CDNA4_ERROR_ADMISSION            */
+  int unit;           /* index of the launch unit the request ran in (-1: none)       */
+  int unit_size;      /* requests in that unit                                        */
+  int pad;
+  double start_s;     /* launched, seconds after the run start                        */
+  double done_s;      /* completion seen by the server, seconds after the run start   */
+} raja_cdna4_result;
+
+typedef struct raja_cdna4_server_s* raja_cdna4_server;
+
+/* Creates the server's streams and arena on the current device. */
+int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_server* server);
+/* Runs nreq requests to completion (blocking) and fills results[0..nreq);
+ * *wall_s (optional) is the time from the run start to the last completion. */
+int raja_cdna4_server_run(raja_cdna4_server server, const raja_cdna4_request* requests,
+                          int64_t nreq, raja_cdna4_result* results, double* wall_s);
+int raja_cdna4_server_destroy(raja_cdna4_server server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,34 @@ class RajaError(RuntimeError):
 
 _lib = None
 
+# ---- request serving (raja_cdna4_serve_config / _request / _result) --------------
+
+REQ_SUM_F64 = 0
+REQ_EXCLUSIVE_SCAN_I32 = 1
+REQ_SORT_PAIRS_U32_I32 = 2
+SERVE_BATCHED = 0
+SERVE_PER_REQUEST = 1
+STATUS_ADMISSION = 5
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("streams", ctypes.c_int),
+                ("max_inflight", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("batch_limit", ctypes.c_int64), ("arena_bytes", ctypes.c_int64),
+                ("honor_arrivals", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("pad", ctypes.c_int), ("n", ctypes.c_int64),
+                ("in_", ctypes.c_void_p), ("in_vals", ctypes.c_void_p),
+                ("out", ctypes.c_void_p), ("out_vals", ctypes.c_void_p),
+                ("arrival_s", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("unit", ctypes.c_int), ("unit_size", ctypes.c_int),
+                ("pad", ctypes.c_int), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
 
 def _declare(L):
     vp, i64, i32, u64, dbl = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
@@ -79,6 +107,10 @@ def _declare(L):
         "raja_cdna4_segmented_exclusive_scan": [vp, i32, i32, vp, vp, vp, vp],
         "raja_cdna4_segmented_inclusive_scan": [vp, i32, i32, vp, vp, vp],
         "raja_cdna4_segmented_sort_pairs": [vp, i32, vp, vp, i32, i32, vp],
+        "raja_cdna4_server_create": [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)],
+        "raja_cdna4_server_run": [vp, ctypes.POINTER(Request), i64, ctypes.POINTER(Result),
+                                  ctypes.POINTER(dbl)],
+        "raja_cdna4_server_destroy": [vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -305,6 +337,53 @@ class Segments:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """A request server (raja_cdna4_server_*): arena admission, batching of
+    small requests, large ones alone on round-robin streams."""
+
+    def __init__(self, mode=SERVE_BATCHED, streams=4, max_inflight=8, max_batch=512,
+                 batch_limit=1 << 16, arena_bytes=1 << 30, honor_arrivals=False):
+        self.config = ServeConfig(mode, streams, max_inflight, max_batch, batch_limit,
+                                  arena_bytes, 1 if honor_arrivals else 0, 0)
+        _prep()
+        h = ctypes.c_void_p()
+        _check(lib().raja_cdna4_server_create(ctypes.byref(self.config), ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, requests):
+        """requests: a ctypes array of Request (or a list of them).  Blocks
+        until all complete; returns (results array, wall seconds)."""
+        if not isinstance(requests, ctypes.Array):
+            requests = (Request * len(requests))(*requests)
+        n = len(requests)
+        results = (Result * max(n, 1))()
+        wall = ctypes.c_double(0.0)
+        torch.cuda.synchronize()  # inputs written by torch on its streams are visible
+        _check(lib().raja_cdna4_server_run(self.handle, requests, n, results, ctypes.byref(wall)))
+        return results, wall.value
+
+    def close(self):
+        if self.handle:
+            _check(lib().raja_cdna4_server_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request(kind, n, in_, out, in_vals=None, out_vals=None, arrival_s=0.0):
+    """One Request from raw device addresses (ints) or tensors."""
+    def addr(t):
+        if t is None:
+            return None
+        return t.data_ptr() if isinstance(t, torch.Tensor) else int(t)
+    return Request(kind, 0, int(n), addr(in_), addr(in_vals), addr(out), addr(out_vals),
+                   float(arrival_s))
 
 
 def segmented_reduce(plan, x, op=operators.plus, init=None, out=None):

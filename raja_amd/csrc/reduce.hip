@@ -14,6 +14,7 @@
 // block serialises across the 8 XCDs; the kernel boundary gives the
 // visibility for free.
 #include "reduce_tree.hpp"
+#include "workspace.hpp"
 
 namespace
 {
@@ -21,6 +22,28 @@ using namespace RAJA::cdna4::dev;
 
 constexpr int kFinalWaves = 16;
 constexpr int kFinalMaxGroups = 1024;  // spans up to 2^26 tiles
+
+// The two kernels over caller-provided partials (ntiles values).
+template<typename T, typename Op>
+int launch_reduce_core(const T* x, int64_t n, typename Op::acc_t init, typename Op::acc_t* out,
+                       typename Op::acc_t* partials, hipStream_t s, const char* name)
+{
+  const int64_t ntiles = rc::ceil_div(n, rt::tile_elems<T>());
+  const int64_t max_blocks = (int64_t)rc::num_cus() * 8;
+  rc::TimedScope ts(name, s);
+  if (ntiles > 0)
+  {
+    const unsigned grid = (unsigned)(ntiles < max_blocks ? ntiles : max_blocks);
+    hipLaunchKernelGGL((rt::tiles_kernel<T, Op>), dim3(grid), dim3(rt::kBlock), 0, s, x, n, ntiles,
+                       partials);
+    RC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL((rt::final_kernel<Op, kFinalWaves, kFinalMaxGroups>), dim3(1),
+                     dim3(kFinalWaves * 64), 0, s, partials, (const uint32_t*)nullptr,
+                     (const uint32_t*)nullptr, ntiles, init, out, (const int64_t*)nullptr);
+  RC_HIP(hipGetLastError());
+  return RAJA_CDNA4_SUCCESS;
+}
 
 template<typename T, typename Op>
 int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
@@ -38,21 +61,7 @@ int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
     return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "span too long for one reduction");
   }
   RC_TRY(rc::ensure_tmp(c, (size_t)(ntiles > 0 ? ntiles : 1) * sizeof(acc_t)));
-  const int64_t max_blocks = (int64_t)rc::num_cus() * 8;
-  acc_t* partials = (acc_t*)c->tmp;
-  rc::TimedScope ts(name, s);
-  if (ntiles > 0)
-  {
-    const unsigned grid = (unsigned)(ntiles < max_blocks ? ntiles : max_blocks);
-    hipLaunchKernelGGL((rt::tiles_kernel<T, Op>), dim3(grid), dim3(rt::kBlock), 0, s, x, n, ntiles,
-                       partials);
-    RC_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL((rt::final_kernel<Op, kFinalWaves, kFinalMaxGroups>), dim3(1),
-                     dim3(kFinalWaves * 64), 0, s, partials, (const uint32_t*)nullptr,
-                     (const uint32_t*)nullptr, ntiles, init, out, (const int64_t*)nullptr);
-  RC_HIP(hipGetLastError());
-  return RAJA_CDNA4_SUCCESS;
+  return launch_reduce_core<T, Op>(x, n, init, out, (acc_t*)c->tmp, s, name);
 }
 
 template<typename T>
@@ -119,6 +128,23 @@ int reduce_loc_typed(int is_max, const void* x, int64_t n, const void* init_val,
 }
 
 }  // namespace
+
+namespace rws
+{
+size_t sum_f64_ws_bytes(int64_t n)
+{
+  return rc::align_up((size_t)(rc::ceil_div(n, rt::tile_elems<double>()) + 1) * sizeof(double), 256);
+}
+
+int sum_f64(const double* x, int64_t n, double* out, void* ws, hipStream_t s)
+{
+  if (rc::ceil_div(n, rt::tile_elems<double>()) > (int64_t)kFinalMaxGroups * 64)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "span too long for one reduction");
+  }
+  return launch_reduce_core<double, rk::OpPlus<double>>(x, n, 0.0, out, (double*)ws, s, "reduce");
+}
+}  // namespace rws
 
 extern "C" {
 

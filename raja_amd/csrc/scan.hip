@@ -21,6 +21,7 @@
 #include "ops.hpp"
 #include "runtime.hpp"
 #include "segments.hpp"
+#include "workspace.hpp"
 
 namespace
 {
@@ -289,8 +290,47 @@ __global__ void clear_desc_kernel(uint64_t* p, int64_t n)
   }
 }
 
+// One scan launch.  tiles == nullptr: one span of n elements (ntiles
+// tiles); otherwise SEG mode over ntiles tile records.  desc holds ntiles
+// descriptors; with `clear` they are zeroed first (required whenever the
+// memory may hold this epoch tag from an earlier use).
+template<typename T, typename Op, bool EXCLUSIVE>
+int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
+                     const rs::TileRec* tiles, int64_t ntiles, void* desc, uint32_t epoch_tag,
+                     bool clear)
+{
+  const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
+  if (clear)
+  {
+    const int64_t words = ntiles * (int64_t)sizeof(Desc<T>) / 8;
+    hipLaunchKernelGGL(clear_desc_kernel, dim3((unsigned)rc::ceil_div(words, 256 * 16) + 1),
+                       dim3(256), 0, s, (uint64_t*)desc, words);
+  }
+  rc::TimedScope ts(name, s);
+  if (tiles)
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, true>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag, tiles);
+  }
+  else if (aligned)
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, false>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag,
+                       (const rs::TileRec*)nullptr);
+  }
+  else
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, false>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag,
+                       (const rs::TileRec*)nullptr);
+  }
+  RC_HIP(hipGetLastError());
+  return RAJA_CDNA4_SUCCESS;
+}
+
 // table == nullptr: one span of n elements; otherwise the tiles of a
-// segment plan (n = total elements of the plan).
+// segment plan (n = total elements of the plan).  Descriptors come from the
+// stream's scratch.
 template<typename T, typename Op, bool EXCLUSIVE>
 int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
                 const rs::TileTable* table = nullptr)
@@ -305,7 +345,6 @@ int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const cha
   rc::StreamCtx* c = nullptr;
   RC_TRY(rc::get_ctx(s, &c));
   RC_TRY(rc::ensure_scan_desc(c, (size_t)ntiles * sizeof(Desc<T>)));
-  const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
   // Descriptors carry the call's epoch; stale ones from earlier calls on this
   // stream never match.  Under graph capture the epoch is frozen into the
   // graph, so the descriptors are cleared by a kernel node instead.
@@ -314,33 +353,9 @@ int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const cha
   const uint32_t epoch_tag = c->scan_epoch << 2;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   RC_HIP(hipStreamIsCapturing(s, &cap));
-  if (cap != hipStreamCaptureStatusNone)
-  {
-    const int64_t words = ntiles * (int64_t)sizeof(Desc<T>) / 8;
-    hipLaunchKernelGGL(clear_desc_kernel, dim3((unsigned)rc::ceil_div(words, 256 * 16) + 1),
-                       dim3(256), 0, s, (uint64_t*)c->scan_desc, words);
-  }
-  rc::TimedScope ts(name, s);
-  if (table)
-  {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, true>), dim3((unsigned)ntiles),
-                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag,
-                       (const rs::TileRec*)table->d_tiles);
-  }
-  else if (aligned)
-  {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, false>), dim3((unsigned)ntiles),
-                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag,
-                       (const rs::TileRec*)nullptr);
-  }
-  else
-  {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, false>), dim3((unsigned)ntiles),
-                       dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)c->scan_desc, epoch_tag,
-                       (const rs::TileRec*)nullptr);
-  }
-  RC_HIP(hipGetLastError());
-  return RAJA_CDNA4_SUCCESS;
+  return launch_scan_core<T, Op, EXCLUSIVE>(
+      in, out, n, init, s, name, table ? (const rs::TileRec*)table->d_tiles : nullptr, ntiles,
+      c->scan_desc, epoch_tag, cap != hipStreamCaptureStatusNone);
 }
 
 template<typename T, bool EXCLUSIVE>
@@ -442,21 +457,34 @@ int launch_segmented_scan(int dtype, int op, bool exclusive, const void* in, voi
                    : seg_scan_dispatch<false>(dtype, op, in, out, plan, init, s);
 }
 
-int launch_table_exclusive_scan_i32(const int32_t* in, int32_t* out, const rs::TileTable& table,
-                                    int64_t total, int32_t init, hipStream_t s)
-{
-  if (table.tile != ScanShape<int32_t>::TILE)
-  {
-    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "scan table has the wrong tile size");
-  }
-  return launch_scan<int32_t, rk::OpPlus<int32_t>, true>(in, out, total, init, s,
-                                                         "segmented_exclusive_scan", &table);
-}
 }  // namespace rscan
 
-namespace
+namespace rws
 {
-}  // namespace
+size_t exclusive_i32_ws_bytes(int64_t ntiles_or_n, bool is_table)
+{
+  const int64_t ntiles = is_table ? ntiles_or_n : rc::ceil_div(ntiles_or_n, ScanShape<int32_t>::TILE);
+  return rc::align_up((size_t)(ntiles + 1) * sizeof(Desc<int32_t>), 256);
+}
+
+int exclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* ws, hipStream_t s)
+{
+  if (n <= 0) return RAJA_CDNA4_SUCCESS;
+  const int64_t ntiles = rc::ceil_div(n, ScanShape<int32_t>::TILE);
+  return launch_scan_core<int32_t, rk::OpPlus<int32_t>, true>(in, out, n, 0, s, "exclusive_scan",
+                                                               nullptr, ntiles, ws, 1u << 2, true);
+}
+
+int table_exclusive_i32(const int32_t* in_base, int32_t* out_base, const rs::TileRec* d_tiles,
+                        int64_t ntiles, void* ws, hipStream_t s)
+{
+  if (ntiles <= 0) return RAJA_CDNA4_SUCCESS;
+  return launch_scan_core<int32_t, rk::OpPlus<int32_t>, true>(
+      in_base, out_base, 0, 0, s, "segmented_exclusive_scan", d_tiles, ntiles, ws, 1u << 2, true);
+}
+
+int64_t scan_tile_i32() { return ScanShape<int32_t>::TILE; }
+}  // namespace rws
 
 extern "C" {
 

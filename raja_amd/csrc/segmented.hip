@@ -21,6 +21,7 @@
 #include "radix.hpp"
 #include "reduce_tree.hpp"
 #include "segments.hpp"
+#include "workspace.hpp"
 
 namespace rs
 {
@@ -170,8 +171,8 @@ __global__ __launch_bounds__(512) void block_sort_kernel(const K* in_keys, const
   for (int j = 0; j < ITEMS; ++j)
   {
     const int i = wave * WCHUNK + j * wave_size + lane;
-    k[j] = (i < n) ? in_keys[sg.in_start + i] : K(0);
-    if (HAS_VALS) v[j] = (i < n) ? in_vals[sg.in_start + i] : V(0);
+    k[j] = (i < n) ? in_keys[sg.in_keys + i] : K(0);
+    if (HAS_VALS) v[j] = (i < n) ? in_vals[sg.in_vals + i] : V(0);
   }
   const uint64_t lt_mask = lanemask_lt();
   for (int p = 0; p < P; ++p)
@@ -273,7 +274,7 @@ int seg_sort(raja_cdna4_segments_s* p, K* keys, V* vals, int key_dtype, int val_
     const int64_t b = p->offsets[(size_t)g];
     const int64_t len = p->offsets[(size_t)g + 1] - b;
     if (len <= 1) continue;
-    if (len <= S::TILE) small.push_back(rs::SortSeg{b, len, b, b});
+    if (len <= S::TILE) small.push_back(rs::SortSeg{b, b, b, b, len});
     else big.push_back(g);
   }
   if (!small.empty())
@@ -331,50 +332,43 @@ int seg_sort_key(raja_cdna4_segments_s* p, int kd, void* keys, void* vals, int v
 
 }  // namespace
 
-namespace rseg
+namespace rws
 {
+int64_t sum_tile_f64() { return rt::tile_elems<double>(); }
 
-int launch_table_sum_f64(const double* x, const rs::TileTable& table, int64_t nseg, double* out,
-                         const int64_t* d_out_index, hipStream_t s)
+int table_sum_f64(const double* base, const rs::TileRec* d_tiles, int64_t ntiles,
+                  const uint32_t* d_first, const uint32_t* d_count, int64_t nseg,
+                  const int64_t* d_out_index, void* ws, hipStream_t s)
 {
   using Op = rk::OpPlus<double>;
-  if (table.tile != rt::tile_elems<double>() || table.max_count > (uint32_t)kSegMaxGroups * 64)
-  {
-    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad table for the batched sum");
-  }
   if (nseg == 0) return RAJA_CDNA4_SUCCESS;
-  rc::StreamCtx* c = nullptr;
-  RC_TRY(rc::get_ctx(s, &c));
-  RC_TRY(rc::ensure_tmp(c, (size_t)(table.ntiles + 1) * sizeof(double)));
-  double* partials = (double*)c->tmp;
+  double* partials = (double*)ws;
   rc::TimedScope ts("segmented_reduce", s);
-  if (table.ntiles > 0)
+  if (ntiles > 0)
   {
-    hipLaunchKernelGGL((seg_tiles_kernel<double, Op>), dim3((unsigned)table.ntiles), dim3(rt::kBlock),
-                       0, s, x, (const rs::TileRec*)table.d_tiles, partials);
+    hipLaunchKernelGGL((seg_tiles_kernel<double, Op>), dim3((unsigned)ntiles), dim3(rt::kBlock), 0, s,
+                       base, d_tiles, partials);
     RC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL((rt::final_kernel<Op, 1, kSegMaxGroups>), dim3((unsigned)nseg), dim3(64), 0, s,
-                     partials, (const uint32_t*)table.d_first, (const uint32_t*)table.d_count,
-                     (int64_t)0, 0.0, out, d_out_index);
+                     (const double*)partials, d_first, d_count, (int64_t)0, 0.0, (double*)base,
+                     d_out_index);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }
 
-int launch_block_sort_u32_i32(const uint32_t* in_keys, const int32_t* in_vals, uint32_t* out_keys,
-                              int32_t* out_vals, const rs::SortSeg* d_segs, int64_t nsmall,
-                              hipStream_t s)
+int64_t block_sort_max() { return BlockSortShape<uint32_t, uint32_t, true>::TILE; }
+
+int block_sort_u32_i32(const uint32_t* base, const rs::SortSeg* d_segs, int64_t nseg, hipStream_t s)
 {
-  if (nsmall == 0) return RAJA_CDNA4_SUCCESS;
+  if (nseg == 0) return RAJA_CDNA4_SUCCESS;
   rc::TimedScope ts("segmented_block_sort", s);
-  hipLaunchKernelGGL((block_sort_kernel<uint32_t, uint32_t, false, true>), dim3((unsigned)nsmall),
-                     dim3(512), 0, s, in_keys, (const uint32_t*)in_vals, out_keys,
-                     (uint32_t*)out_vals, d_segs);
+  hipLaunchKernelGGL((block_sort_kernel<uint32_t, uint32_t, false, true>), dim3((unsigned)nseg),
+                     dim3(512), 0, s, base, base, (uint32_t*)base, (uint32_t*)base, d_segs);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }
-
-}  // namespace rseg
+}  // namespace rws
 
 extern "C" {
 

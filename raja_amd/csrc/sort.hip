@@ -23,7 +23,7 @@
 #include "ops.hpp"
 #include "radix.hpp"
 #include "runtime.hpp"
-#include "segments.hpp"
+#include "workspace.hpp"
 
 namespace
 {
@@ -364,32 +364,46 @@ struct SortShape
   static constexpr int64_t TILE = (int64_t)BLOCK * ITEMS;
 };
 
-// allow_host_sync: read the histograms back (one stream synchronisation)
-// to skip passes whose digit is the same for every key.
+// Working memory of one sort: alternate key/value buffers, the digit
+// histograms and two descriptor arrays.
+template<typename K, typename V, bool HAS_VALS>
+struct SortWs
+{
+  using S = SortShape<K, V, HAS_VALS>;
+  static constexpr int P = (int)sizeof(K);
+  int64_t n, ntiles, desc_words;
+  size_t keys_bytes, vals_bytes, hist_bytes, desc_bytes;
+  explicit SortWs(int64_t n_) : n(n_)
+  {
+    ntiles = rc::ceil_div(n, S::TILE);
+    desc_words = ntiles * kRowStride;
+    keys_bytes = rc::align_up((size_t)n * sizeof(K), 256);
+    vals_bytes = HAS_VALS ? rc::align_up((size_t)n * sizeof(V), 256) : 0;
+    hist_bytes = rc::align_up((size_t)P * kRadix * sizeof(uint32_t), 256);
+    desc_bytes = 2 * (size_t)desc_words * sizeof(uint32_t);
+  }
+  size_t tmp_bytes() const { return keys_bytes + vals_bytes + hist_bytes; }
+  size_t total() const { return tmp_bytes() + desc_bytes; }
+};
+
+// Sort (in_keys, in_vals) into (keys, vals); in_keys == keys sorts in place.
+// tmp holds SortWs::tmp_bytes(), desc_mem SortWs::desc_bytes (its contents
+// are ignored: the histogram kernel clears the first array and every pass
+// clears the next).  With `pinned` (>= hist bytes of pinned host memory) the
+// histograms are read back -- one stream synchronisation -- to skip passes
+// whose digit is the same for every key.
 template<typename K, typename V, bool HAS_VALS, bool DESC>
-int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s, bool allow_host_sync = true)
+int sort_core(const K* in_keys, const V* in_vals, K* keys, V* vals, int64_t n, hipStream_t s,
+              char* tmp, uint32_t* desc_mem, uint32_t* pinned)
 {
   using S = SortShape<K, V, HAS_VALS>;
   constexpr int P = (int)sizeof(K);
-  if (n > kMaxSortN)
-  {
-    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort length %lld exceeds %lld",
-                         (long long)n, (long long)kMaxSortN);
-  }
-  const int64_t ntiles = rc::ceil_div(n, S::TILE);
-  rc::StreamCtx* c = nullptr;
-  RC_TRY(rc::get_ctx(s, &c));
-  const size_t keys_bytes = rc::align_up((size_t)n * sizeof(K), 256);
-  const size_t vals_bytes = HAS_VALS ? rc::align_up((size_t)n * sizeof(V), 256) : 0;
-  const size_t hist_bytes = (size_t)P * kRadix * sizeof(uint32_t);
-  RC_TRY(rc::ensure_tmp(c, keys_bytes + vals_bytes + hist_bytes));
-  const int64_t desc_words = ntiles * kRowStride;
-  RC_TRY(rc::ensure_sort_desc(c, 2 * (size_t)desc_words * sizeof(uint32_t)));
-  char* tmp = (char*)c->tmp;
+  const SortWs<K, V, HAS_VALS> w(n);
   K* alt_keys = (K*)tmp;
-  V* alt_vals = HAS_VALS ? (V*)(tmp + keys_bytes) : nullptr;
-  uint32_t* hist = (uint32_t*)(tmp + keys_bytes + vals_bytes);
-  uint32_t* desc[2] = {(uint32_t*)c->sort_desc, (uint32_t*)c->sort_desc + desc_words};
+  V* alt_vals = HAS_VALS ? (V*)(tmp + w.keys_bytes) : nullptr;
+  uint32_t* hist = (uint32_t*)(tmp + w.keys_bytes + w.vals_bytes);
+  uint32_t* desc[2] = {desc_mem, desc_mem + w.desc_words};
+  const bool inplace = (const void*)in_keys == (const void*)keys;
 
   {
     rc::TimedScope ts("radix_hist", s);
@@ -397,40 +411,51 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s, bool allow_host_sync =
     const int64_t want = rc::ceil_div(n, (int64_t)256 * 64);
     const int64_t cap = (int64_t)rc::num_cus() * 4;
     const unsigned grid = (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
-    hipLaunchKernelGGL((hist_kernel<K, DESC, P>), dim3(grid), dim3(256), 0, s, keys, n, hist,
-                       desc[0], desc_words);
+    hipLaunchKernelGGL((hist_kernel<K, DESC, P>), dim3(grid), dim3(256), 0, s, in_keys, n, hist,
+                       desc[0], w.desc_words);
     RC_HIP(hipGetLastError());
   }
   bool active[P];
+  int npass = P;
   for (int p = 0; p < P; ++p) active[p] = true;
-  if (allow_host_sync && n >= (int64_t)1 << 18)
+  if (pinned && n >= (int64_t)1 << 18)
   {
-    uint32_t* h = (uint32_t*)c->pinned;
-    RC_HIP(hipMemcpyAsync(h, hist, hist_bytes, hipMemcpyDeviceToHost, s));
+    RC_HIP(hipMemcpyAsync(pinned, hist, (size_t)P * kRadix * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
     RC_HIP(hipStreamSynchronize(s));
     for (int p = 0; p < P; ++p)
     {
       for (int dd = 0; dd < kRadix; ++dd)
       {
-        if ((int64_t)h[p * kRadix + dd] == n) active[p] = false;
+        if ((int64_t)pinned[p * kRadix + dd] == n)
+        {
+          active[p] = false;
+          --npass;
+        }
       }
     }
   }
-  K* ksrc = keys;
-  K* kdst = alt_keys;
-  V* vsrc = vals;
-  V* vdst = alt_vals;
+  // Out of place, an even pass count starts into the alternate buffers and
+  // an odd one into the outputs, so the last pass always lands in the
+  // outputs.  In place, passes start into the alternates and an odd count
+  // ends with one copy back.
+  const K* ksrc = in_keys;
+  const V* vsrc = in_vals;
+  bool to_alt = inplace || (npass % 2 == 0);
   int q = 0;
   for (int p = 0; p < P; ++p)
   {
     if (!active[p]) continue;
+    K* kdst = to_alt ? alt_keys : keys;
+    V* vdst = to_alt ? alt_vals : vals;
     rc::TimedScope ts("onesweep_pass", s);
     hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::ITEMS>),
-                       dim3((unsigned)ntiles), dim3(S::BLOCK), 0, s, ksrc, kdst, vsrc, vdst, n, 8 * p,
-                       hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
+                       dim3((unsigned)w.ntiles), dim3(S::BLOCK), 0, s, (K*)ksrc, kdst, (V*)vsrc, vdst,
+                       n, 8 * p, hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
     RC_HIP(hipGetLastError());
-    K* tk = ksrc; ksrc = kdst; kdst = tk;
-    V* tv = vsrc; vsrc = vdst; vdst = tv;
+    ksrc = kdst;
+    vsrc = vdst;
+    to_alt = !to_alt;
     ++q;
   }
   if (ksrc != keys)
@@ -442,6 +467,24 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s, bool allow_host_sync =
     }
   }
   return RAJA_CDNA4_SUCCESS;
+}
+
+// In-place sort with the stream's scratch.
+template<typename K, typename V, bool HAS_VALS, bool DESC>
+int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
+{
+  if (n > kMaxSortN)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort length %lld exceeds %lld",
+                         (long long)n, (long long)kMaxSortN);
+  }
+  rc::StreamCtx* c = nullptr;
+  RC_TRY(rc::get_ctx(s, &c));
+  const SortWs<K, V, HAS_VALS> w(n);
+  RC_TRY(rc::ensure_tmp(c, w.tmp_bytes()));
+  RC_TRY(rc::ensure_sort_desc(c, w.desc_bytes));
+  return sort_core<K, V, HAS_VALS, DESC>(keys, vals, keys, vals, n, s, (char*)c->tmp,
+                                         (uint32_t*)c->sort_desc, (uint32_t*)c->pinned);
 }
 
 template<typename K>
@@ -467,14 +510,30 @@ int sort_key_type(void* keys, void* vals, int val_bytes, int64_t n, int descendi
 
 }  // namespace
 
-namespace rsort
+namespace rws
 {
-int sort_pairs_u32_i32_async(uint32_t* keys, int32_t* vals, int64_t n, hipStream_t s)
+size_t sort_u32_i32_ws_bytes(int64_t n)
 {
-  if (n <= 1) return RAJA_CDNA4_SUCCESS;
-  return sort_impl<uint32_t, uint32_t, true, false>(keys, (uint32_t*)vals, n, s, false);
+  return SortWs<uint32_t, uint32_t, true>(n).total();
 }
-}  // namespace rsort
+
+int sort_pairs_u32_i32(const uint32_t* in_keys, const int32_t* in_vals, uint32_t* out_keys,
+                       int32_t* out_vals, int64_t n, void* ws, hipStream_t s)
+{
+  if (n <= 0) return RAJA_CDNA4_SUCCESS;
+  if (n > kMaxSortN) return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort too long");
+  if (n == 1)
+  {
+    RC_HIP(hipMemcpyAsync(out_keys, in_keys, 4, hipMemcpyDeviceToDevice, s));
+    RC_HIP(hipMemcpyAsync(out_vals, in_vals, 4, hipMemcpyDeviceToDevice, s));
+    return RAJA_CDNA4_SUCCESS;
+  }
+  const SortWs<uint32_t, uint32_t, true> w(n);
+  return sort_core<uint32_t, uint32_t, true, false>(in_keys, (const uint32_t*)in_vals, out_keys,
+                                                    (uint32_t*)out_vals, n, s, (char*)ws,
+                                                    (uint32_t*)((char*)ws + w.tmp_bytes()), nullptr);
+}
+}  // namespace rws
 
 extern "C" {
 

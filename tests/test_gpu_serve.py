@@ -1,0 +1,183 @@
+"""GPU parity of the request server: every request of a mixed stream (FP64
+sums, int32 exclusive scans, (uint32, int32) stable sort_pairs) checked
+against the oracle, and the serving identity -- a request's result has the
+same bits in a batch, alone through the span entry points, and in the
+per-request mode."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+EPS = 2.0 ** -53
+
+
+class Pools:
+    """Inputs and outputs of a request mix, each request on its own slices."""
+
+    def __init__(self, raja, kinds, sizes, seed):
+        self.kinds, self.sizes = kinds, sizes
+        nsum = sum(int(n) for k, n in zip(kinds, sizes) if k == raja.REQ_SUM_F64)
+        nscan = sum(int(n) for k, n in zip(kinds, sizes) if k == raja.REQ_EXCLUSIVE_SCAN_I32)
+        nsort = sum(int(n) for k, n in zip(kinds, sizes) if k == raja.REQ_SORT_PAIRS_U32_I32)
+        # one spare element keeps every pool non-empty
+        self.x = torch.empty(nsum + 1, dtype=torch.float64, device=DEV)
+        raja.fill(raja.FILL_F64_NORMAL, self.x, seed=seed)
+        self.sums = torch.full((len(kinds),), float("nan"), dtype=torch.float64, device=DEV)
+        self.scan_in = torch.empty(nscan + 1, dtype=torch.int32, device=DEV)
+        raja.fill(raja.FILL_I32, self.scan_in, seed=seed + 1, a=0, b=16)
+        self.scan_out = torch.full((nscan + 1,), -7, dtype=torch.int32, device=DEV)
+        keys = torch.empty(nsort + 1, dtype=torch.int32, device=DEV)
+        # narrow key range: many duplicates, so stability is exercised
+        raja.fill(raja.FILL_I32, keys, seed=seed + 2, a=0, b=4096)
+        self.keys = keys
+        self.vals = torch.arange(nsort + 1, dtype=torch.int32, device=DEV)
+        self.okeys = torch.full((nsort + 1,), -1, dtype=torch.int32, device=DEV)
+        self.ovals = torch.full((nsort + 1,), -1, dtype=torch.int32, device=DEV)
+        self.spans = []
+        a = b = c = 0
+        for k, n in zip(kinds, sizes):
+            n = int(n)
+            if k == raja.REQ_SUM_F64:
+                self.spans.append((a, n)); a += n
+            elif k == raja.REQ_EXCLUSIVE_SCAN_I32:
+                self.spans.append((b, n)); b += n
+            else:
+                self.spans.append((c, n)); c += n
+
+    def requests(self, raja, arrivals=None):
+        reqs = []
+        for i, (k, (o, n)) in enumerate(zip(self.kinds, self.spans)):
+            t = 0.0 if arrivals is None else float(arrivals[i])
+            if k == raja.REQ_SUM_F64:
+                reqs.append(raja.request(k, n, self.x[o:], self.sums[i:], arrival_s=t))
+            elif k == raja.REQ_EXCLUSIVE_SCAN_I32:
+                reqs.append(raja.request(k, n, self.scan_in[o:], self.scan_out[o:], arrival_s=t))
+            else:
+                reqs.append(raja.request(k, n, self.keys[o:], self.okeys[o:], self.vals[o:],
+                                         self.ovals[o:], arrival_s=t))
+        return reqs
+
+    def reset_outputs(self):
+        self.sums.fill_(float("nan"))
+        self.scan_out.fill_(-7)
+        self.okeys.fill_(-1)
+        self.ovals.fill_(-1)
+
+    def snapshot(self):
+        torch.cuda.synchronize()
+        return (self.sums.cpu().numpy().copy(), self.scan_out.cpu().numpy().copy(),
+                self.okeys.cpu().numpy().copy(), self.ovals.cpu().numpy().copy())
+
+
+def check_against_oracle(raja, orc, pools, snap):
+    sums, scan_out, okeys, ovals = snap
+    x = pools.x.cpu().numpy()
+    sin = pools.scan_in.cpu().numpy()
+    keys = pools.keys.cpu().numpy().view(np.uint32)
+    vals = pools.vals.cpu().numpy()
+    for i, (k, (o, n)) in enumerate(zip(pools.kinds, pools.spans)):
+        if k == raja.REQ_SUM_F64:
+            seg = x[o:o + n]
+            if n == 0:
+                assert sums[i] == 0.0, i
+                continue
+            kn = orc.kn_sum_f64(seg)
+            bound = 2 * max(1, math.ceil(math.log2(max(n, 2)))) * EPS * float(np.abs(seg).sum())
+            assert abs(sums[i] - kn) <= bound, (i, n)
+        elif k == raja.REQ_EXCLUSIVE_SCAN_I32:
+            want = orc.seq_exclusive(sin[o:o + n], orc.PLUS, 0)
+            assert np.array_equal(scan_out[o:o + n], want), (i, n)
+        else:
+            wk, wv = orc.seq_stable_sort_pairs(keys[o:o + n].copy(), vals[o:o + n].copy())
+            assert np.array_equal(okeys[o:o + n].view(np.uint32), wk), (i, n)
+            assert np.array_equal(ovals[o:o + n], wv), (i, n)
+
+
+def mix(raja, nreq, seed, max_log2):
+    rng = np.random.default_rng(seed)
+    u = rng.random(nreq)
+    kinds = np.where(u < 0.4, raja.REQ_SUM_F64,
+                     np.where(u < 0.7, raja.REQ_EXCLUSIVE_SCAN_I32, raja.REQ_SORT_PAIRS_U32_I32))
+    sizes = np.floor(2.0 ** (rng.random(nreq) * max_log2)).astype(np.int64)
+    return [int(k) for k in kinds], sizes
+
+
+def test_serve_mix_matches_oracle_and_modes_agree(raja, orc):
+    kinds, sizes = mix(raja, 300, seed=11, max_log2=18)
+    sizes[:6] = [0, 1, 0, 1, 8192, 8193]
+    kinds[:6] = [raja.REQ_SUM_F64, raja.REQ_SORT_PAIRS_U32_I32, raja.REQ_EXCLUSIVE_SCAN_I32,
+                 raja.REQ_EXCLUSIVE_SCAN_I32, raja.REQ_SORT_PAIRS_U32_I32,
+                 raja.REQ_SORT_PAIRS_U32_I32]
+    pools = Pools(raja, kinds, sizes, seed=21)
+    print("serve mix: 300 requests, batched", flush=True)
+    srv = raja.Server(raja.SERVE_BATCHED, streams=4, max_inflight=8, max_batch=64,
+                      batch_limit=1 << 16, arena_bytes=1 << 30)
+    res, wall = srv.run(pools.requests(raja))
+    assert all(r.status == 0 for r in res)
+    assert max(r.unit_size for r in res) > 1  # some requests were batched
+    batched = pools.snapshot()
+    check_against_oracle(raja, orc, pools, batched)
+    srv.close()
+
+    print("serve mix: per-request mode", flush=True)
+    pools.reset_outputs()
+    srv = raja.Server(raja.SERVE_PER_REQUEST, streams=1, max_inflight=4, max_batch=1,
+                      batch_limit=0, arena_bytes=1 << 30)
+    res, _ = srv.run(pools.requests(raja))
+    assert all(r.status == 0 and r.unit_size == 1 for r in res)
+    single = pools.snapshot()
+    srv.close()
+    for a, b in zip(batched, single):
+        assert np.array_equal(a, b, equal_nan=False)
+
+    # a sum served in a batch has the bits of the span reduction of its range
+    for i, (k, (o, n)) in enumerate(zip(kinds, pools.spans)):
+        if k == raja.REQ_SUM_F64 and i % 5 == 0:
+            alone = raja.reduce_sum(pools.x[o:o + n], 0.0).item()
+            assert alone == batched[0][i], (i, n)
+
+
+def test_serve_admission_rejects_oversized(raja, orc):
+    kinds = [raja.REQ_SUM_F64, raja.REQ_SORT_PAIRS_U32_I32, raja.REQ_EXCLUSIVE_SCAN_I32,
+             raja.REQ_SORT_PAIRS_U32_I32]
+    sizes = np.array([5000, 1 << 17, 3000, 100], dtype=np.int64)
+    pools = Pools(raja, kinds, sizes, seed=31)
+    # 256 KiB arena: the 2^17-pair sort needs about 1 MiB of alternates
+    srv = raja.Server(raja.SERVE_BATCHED, streams=2, max_inflight=4, max_batch=8,
+                      batch_limit=1 << 16, arena_bytes=256 << 10)
+    res, _ = srv.run(pools.requests(raja))
+    st = [r.status for r in res]
+    assert st == [0, raja.STATUS_ADMISSION, 0, 0]
+    snap = pools.snapshot()
+    srv.close()
+    # the rejected request left its outputs untouched; the others are right
+    o, n = pools.spans[1]
+    assert (snap[2][o:o + n] == -1).all()
+    keep = [0, 2, 3]
+    sub = Pools.__new__(Pools)
+    sub.kinds = [kinds[i] for i in keep]
+    sub.spans = [pools.spans[i] for i in keep]
+    sub.x, sub.scan_in, sub.keys, sub.vals = pools.x, pools.scan_in, pools.keys, pools.vals
+    check_against_oracle(raja, orc, sub, (snap[0][keep], snap[1], snap[2], snap[3]))
+
+
+def test_serve_arrivals_and_small_arena(raja, orc):
+    """Poisson arrivals honoured; an arena that holds few units at a time
+    forces waiting on retirements; results stay exact."""
+    kinds, sizes = mix(raja, 200, seed=41, max_log2=15)
+    rng = np.random.default_rng(42)
+    arrivals = np.cumsum(rng.exponential(20e-6, len(kinds)))
+    pools = Pools(raja, kinds, sizes, seed=51)
+    srv = raja.Server(raja.SERVE_BATCHED, streams=3, max_inflight=6, max_batch=16,
+                      batch_limit=1 << 16, arena_bytes=2 << 20, honor_arrivals=True)
+    res, wall = srv.run(pools.requests(raja, arrivals))
+    assert all(r.status == 0 for r in res)
+    for r, t in zip(res, arrivals):
+        assert r.start_s >= t - 1e-9 and r.done_s >= r.start_s
+    assert wall >= arrivals[-1]
+    check_against_oracle(raja, orc, pools, pools.snapshot())
+    srv.close()
