@@ -18,6 +18,7 @@ this host's cores, scaled to the same GB/s definition.
 import argparse
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -57,6 +58,7 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ops", action="store_true", help="skip the secondary ops")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-serving stream")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -145,10 +147,15 @@ def main():
         del keys0, vals0
         ops = secondary_ops(R, torch, np, dev, stream, n)
 
+    serve = None
+    if not args.no_serve:
+        serve = serving(R, torch, np, dev)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np)
 
+    traffic, traffic_src = committed_traffic(args.log2n)
     line = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -175,13 +182,31 @@ def main():
             "frac": round(pass_gbs / PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": n * 16,
             "avg_launch_ms": round(pass_ms, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "hist_kernel_ms": round(hist["total_ms"] / max(1, hist["count"]), 4),
         },
         "cpu_baseline": cpu,
         "ops": ops,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
+
+
+TRAFFIC_FILE = os.path.join("profiles", "onesweep_pass_sort_pairs_2p28_hbm_traffic.md")
+
+
+def committed_traffic(log2n):
+    """HBM bytes per onesweep pass at N=2^28, from the committed FETCH_SIZE /
+    WRITE_SIZE summary (counter runs scaled by a plain copy of known size)."""
+    if log2n != 28:
+        return None, None
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            m = re.search(r"^traffic_bytes_per_launch: (\d+)$", f.read(), re.M)
+    except OSError:
+        return None, None
+    return (int(m.group(1)), TRAFFIC_FILE) if m else (None, None)
 
 
 def secondary_ops(R, torch, np, dev, stream, n):
@@ -232,6 +257,118 @@ def secondary_ops(R, torch, np, dev, stream, n):
     rec("sort_pairs_u32_dup12", t, n * (16 * 2 + 4),
         {"passes_executed": 2, "bytes_per_elem": 36, "pass_count_bound_full_P4_GBs_equiv":
          round(n * 68 / (statistics.median(t) * 1e-3) / 1e9, 1)})
+    return out
+
+
+def serve_mix(np, nreq, seed, lo_log2, span_log2):
+    """Request kinds 40% FP64 sum / 30% int32 exclusive scan / 30% sort_pairs,
+    sizes floor(2^(lo + span*u))."""
+    rng = np.random.default_rng(seed)
+    u = rng.random(nreq)
+    kinds = np.where(u < 0.4, 0, np.where(u < 0.7, 1, 2))
+    sizes = np.floor(2.0 ** (lo_log2 + span_log2 * rng.random(nreq))).astype(np.int64)
+    return kinds, sizes, rng
+
+
+def serving(R, torch, np, dev, nreq=4000):
+    """The request stream: saturation req/s (every request waiting at t=0),
+    batched vs one launch sequence per request, and p50/p99 latency under
+    Poisson arrivals below the batched saturation rate."""
+    nmax, ring = 1 << 24, 1 << 28
+    x = torch.empty(nmax, dtype=torch.float64, device=dev)
+    R.fill(R.FILL_F64_NORMAL, x, seed=5)
+    si = torch.empty(nmax, dtype=torch.int32, device=dev)
+    R.fill(R.FILL_I32, si, seed=3, a=0, b=16)
+    sk = torch.empty(nmax, dtype=torch.uint32, device=dev)
+    R.fill(R.FILL_U32_BITS, sk, seed=1, a=32)
+    sv = torch.arange(nmax, dtype=torch.int32, device=dev)
+    sums = torch.empty(nreq, dtype=torch.float64, device=dev)
+    so = torch.empty(ring, dtype=torch.int32, device=dev)
+    ok_ = torch.empty(ring, dtype=torch.uint32, device=dev)
+    ov = torch.empty(ring, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def build(kinds, sizes, rng, arrivals=None):
+        reqs = (R.Request * len(kinds))()
+        cur = [0, 0]
+        nbytes = 0
+        for i, (k, n) in enumerate(zip(kinds, sizes)):
+            n = int(n)
+            src = int(rng.integers(0, (nmax - n) // 64 + 1)) * 64
+            t = 0.0 if arrivals is None else float(arrivals[i])
+            if k == 0:
+                reqs[i] = R.request(0, n, x.data_ptr() + 8 * src, sums.data_ptr() + 8 * i, arrival_s=t)
+                nbytes += 8 * n
+                continue
+            c = 0 if k == 1 else 1
+            if cur[c] + n > ring:
+                cur[c] = 0
+            o = cur[c]
+            cur[c] += (n + 63) // 64 * 64
+            if k == 1:
+                reqs[i] = R.request(1, n, si.data_ptr() + 4 * src, so.data_ptr() + 4 * o, arrival_s=t)
+                nbytes += 8 * n
+            else:
+                reqs[i] = R.request(2, n, sk.data_ptr() + 4 * src, ok_.data_ptr() + 4 * o,
+                                    sv.data_ptr() + 4 * src, ov.data_ptr() + 4 * o, arrival_s=t)
+                nbytes += 68 * n
+        return reqs, nbytes
+
+    def lat(res, arrivals):
+        d = np.array([r.done_s for r in res]) - np.asarray(arrivals)
+        return float(np.percentile(d, 50) * 1e3), float(np.percentile(d, 99) * 1e3)
+
+    cfg = dict(streams=4, max_inflight=8, max_batch=512, batch_limit=1 << 16, arena_bytes=64 << 30)
+    out = {"workload": "%d requests: 40%% FP64 sum, 30%% int32 exclusive scan, 30%% sort_pairs "
+                       "(uint32, int32); n = floor(2^(10+14u)); inputs resident; 64 GB arena" % nreq}
+    servers = {
+        "batched": R.Server(R.SERVE_BATCHED, **cfg),
+        "per_request": R.Server(R.SERVE_PER_REQUEST, 1, 8, 1, 0, 64 << 30),
+    }
+    kinds, sizes, rng = serve_mix(np, nreq, 7, 10, 14)
+    reqs, nbytes = build(kinds, sizes, rng)
+    sat = {}
+    for name, srv in servers.items():
+        srv.run(reqs[:200])  # warm-up
+        walls = []
+        for _ in range(3):
+            res, wall = srv.run(reqs)
+            assert all(r.status == 0 for r in res)
+            walls.append(wall)
+        w = statistics.median(walls)
+        sat[name] = nreq / w
+        out["saturation_" + name] = {"req_per_s": round(nreq / w, 1), "wall_s": round(w, 4),
+                                     "GB_per_s": round(nbytes / w / 1e9, 1)}
+        log("bench: serve saturation %-12s %9.1f req/s  %.1f GB/s" % (name, nreq / w, nbytes / w / 1e9))
+    out["saturation_speedup"] = round(sat["batched"] / sat["per_request"], 3)
+
+    # small-request stream (n in [2^10, 2^16]): launch-latency bound
+    kinds_s, sizes_s, rng_s = serve_mix(np, nreq, 8, 10, 6)
+    reqs_s, nbytes_s = build(kinds_s, sizes_s, rng_s)
+    sat_s = {}
+    for name, srv in servers.items():
+        srv.run(reqs_s[:200])
+        walls = [srv.run(reqs_s)[1] for _ in range(3)]
+        w = statistics.median(walls)
+        sat_s[name] = nreq / w
+        out["small_saturation_" + name] = {"req_per_s": round(nreq / w, 1), "wall_s": round(w, 4)}
+        log("bench: serve small-request saturation %-12s %9.1f req/s" % (name, nreq / w))
+    out["small_saturation_speedup"] = round(sat_s["batched"] / sat_s["per_request"], 3)
+
+    # Poisson arrivals at 60% of the batched saturation rate
+    lam = 0.6 * sat["batched"]
+    arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / lam, nreq))
+    reqs_p, _ = build(kinds, sizes, np.random.default_rng(9), arr)
+    for name, args_ in (("batched", (R.SERVE_BATCHED, 4, 8, 512, 1 << 16, 64 << 30, True)),
+                        ("per_request", (R.SERVE_PER_REQUEST, 1, 8, 1, 0, 64 << 30, True))):
+        servers[name].close()
+        srv = R.Server(*args_)
+        res, wall = srv.run(reqs_p)
+        p50, p99 = lat(res, arr)
+        out["poisson_" + name] = {"offered_req_per_s": round(lam, 1), "req_per_s": round(nreq / wall, 1),
+                                  "p50_ms": round(p50, 4), "p99_ms": round(p99, 4)}
+        log("bench: serve poisson %.0f req/s %-12s p50 %.3f ms  p99 %.3f ms" % (lam, name, p50, p99))
+        srv.close()
     return out
 
 
