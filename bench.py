@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ops", action="store_true", help="skip the secondary ops")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving stream")
+    ap.add_argument("--no-halo", action="store_true", help="skip the WorkGroup halo exchange")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -147,6 +148,10 @@ def main():
         del keys0, vals0
         ops = secondary_ops(R, torch, np, dev, stream, n)
 
+    halo = None
+    if not args.no_halo:
+        halo = halo_exchange()
+
     serve = None
     if not args.no_serve:
         serve = serving(R, torch, np, dev)
@@ -189,6 +194,7 @@ def main():
         "cpu_baseline": cpu,
         "ops": ops,
         "serving": serve,
+        "workgroup_halo": halo,
     }
     print(json.dumps(line), flush=True)
 
@@ -257,6 +263,85 @@ def secondary_ops(R, torch, np, dev, stream, n):
     rec("sort_pairs_u32_dup12", t, n * (16 * 2 + 4),
         {"passes_executed": 2, "bytes_per_elem": 36, "pass_count_bound_full_P4_GBs_equiv":
          round(n * 68 / (statistics.median(t) * 1e-3) / 1e9, 1)})
+    del k0, v0, k, v
+    out.update(segmented_ops(R, torch, np, dev, stream, rec))
+    return out
+
+
+def segmented_ops(R, torch, np, dev, stream, rec):
+    """20000 segments of length clamp(floor(2^(16u)), 1, 65536), u uniform
+    (seed 6): batched FP64 sum and int32 exclusive scan over one plan, and
+    the same work as one C-ABI call per segment issued from a loop."""
+    import ctypes
+    nseg = 20000
+    u = np.random.default_rng(6).random(nseg)
+    lens = np.clip(np.floor(2.0 ** (16 * u)), 1, 1 << 16).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(off[-1])
+    plan = R.Segments(off)
+    x = torch.empty(total, dtype=torch.float64, device=dev)
+    R.fill(R.FILL_F64_NORMAL, x, seed=6)
+    sums = torch.empty(nseg, dtype=torch.float64, device=dev)
+    xi = torch.empty(total, dtype=torch.int32, device=dev)
+    R.fill(R.FILL_I32, xi, seed=6, a=0, b=16)
+    oi = torch.empty_like(xi)
+    R.segmented_reduce(plan, x, R.operators.plus, 0.0, sums)  # builds the plan's tables
+    R.segmented_exclusive_scan(plan, xi, oi, R.operators.plus, 0)
+    out = {}
+    t = event_time(lambda: R.segmented_reduce(plan, x, R.operators.plus, 0.0, sums), stream, reps=9)
+    rec("segmented_sum_f64_20000", t, 8 * total + 8 * nseg, {"elements": total})
+    tb = event_time(lambda: R.segmented_exclusive_scan(plan, xi, oi, R.operators.plus, 0), stream, reps=9)
+    rec("segmented_exclusive_scan_i32_20000", tb, 8 * total, {"elements": total})
+
+    L = R.lib()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    zd, zi = ctypes.c_double(0.0), ctypes.c_int32(0)
+    xp, sp_out, ip, op_ = x.data_ptr(), sums.data_ptr(), xi.data_ptr(), oi.data_ptr()
+
+    def per_seg_sum():
+        f = L.raja_cdna4_reduce
+        for g in range(nseg):
+            f(5, 0, xp + 8 * int(off[g]), int(lens[g]), ctypes.byref(zd), sp_out + 8 * g, sp)
+
+    def per_seg_scan():
+        f = L.raja_cdna4_exclusive_scan
+        for g in range(nseg):
+            f(0, 0, ip + 4 * int(off[g]), op_ + 4 * int(off[g]), int(lens[g]), ctypes.byref(zi), sp)
+    t1 = event_time(per_seg_sum, stream, reps=3)
+    t2 = event_time(per_seg_scan, stream, reps=3)
+    rec("per_segment_calls_sum_f64_20000", t1, 8 * total + 8 * nseg)
+    rec("per_segment_calls_exclusive_scan_i32_20000", t2, 8 * total)
+    out_speed = {
+        "segmented_speedup_vs_per_segment_calls": {
+            "sum": round(statistics.median(t1) / statistics.median(t), 2),
+            "exclusive_scan": round(statistics.median(t2) / statistics.median(tb), 2)}}
+    log("bench: segmented speedup vs per-segment calls: %s" % out_speed)
+    plan.close()
+    out.update(out_speed)
+    return out
+
+
+def halo_exchange():
+    """The WorkGroup halo exchange (256^3 grid, halo 2, 10 FP64 variables,
+    260 loops per phase) run by raja_amd/bin/halo_exchange in a child
+    process: per-loop forall, ordered WorkGroup, unordered tile WorkGroup."""
+    import subprocess
+    exe = os.path.join(ROOT, "raja_amd", "bin", "halo_exchange")
+    if not os.path.exists(exe):
+        log("bench: raja_amd/bin/halo_exchange not built; halo skipped")
+        return None
+    r = subprocess.run([exe, "256", "256", "256", "2", "10", "5"], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("halo_exchange failed: " + r.stdout + r.stderr)
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    ph = lambda v, k: out[v]["pack_%s_ms" % k] + out[v]["unpack_%s_ms" % k]
+    out["speedup_unordered_vs_ordered_device"] = round(ph("ordered", "device") / ph("unordered", "device"), 3)
+    out["speedup_unordered_vs_forall_host"] = round(ph("forall", "host") / ph("unordered", "host"), 3)
+    for v in ("forall", "ordered", "unordered"):
+        log("bench: halo %-9s pack %.4f ms unpack %.4f ms (device), %.4f + %.4f ms (host), %.1f / %.1f GB/s"
+            % (v, out[v]["pack_device_ms"], out[v]["unpack_device_ms"], out[v]["pack_host_ms"],
+               out[v]["unpack_host_ms"], out[v]["pack_GBs"], out[v]["unpack_GBs"]))
     return out
 
 

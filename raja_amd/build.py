@@ -75,6 +75,29 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+EXAMPLES = os.path.join(HERE, "examples")
+BIN = os.path.join(HERE, "bin")
+
+
+def build_examples():
+    """Example programs of the C++ front-end (raja_amd/examples/*.cpp) into
+    raja_amd/bin/, linked to libraja_cdna4.so through an $ORIGIN rpath."""
+    os.makedirs(BIN, exist_ok=True)
+    outs = []
+    for name in sorted(os.listdir(EXAMPLES)):
+        if not name.endswith(".cpp"):
+            continue
+        out = os.path.join(BIN, name[:-4])
+        cmd = [HIPCC, "-std=c++17", "-O3", "-ffp-contract=off", "-Wall"] + DEVICE + [
+            "-I" + os.path.join(ROOT, "include"), os.path.join(EXAMPLES, name),
+            "-L" + HERE, "-lraja_cdna4", "-Wl,-rpath,$ORIGIN/..", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("%s build failed:\n%s%s" % (name, r.stdout, r.stderr))
+        outs.append(out)
+    return outs
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
     if os.path.exists(LIB):
@@ -83,3 +106,4 @@ def clean():
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print("\n".join(build_examples()))

@@ -85,8 +85,8 @@ def check_against_oracle(raja, orc, pools, snap):
             if n == 0:
                 assert sums[i] == 0.0, i
                 continue
-            kn = orc.kn_sum_f64(seg)
-            bound = 2 * max(1, math.ceil(math.log2(max(n, 2)))) * EPS * float(np.abs(seg).sum())
+            kn, ab = orc.kn_sum_f64(seg)
+            bound = 2 * max(1, math.ceil(math.log2(max(n, 2)))) * EPS * ab
             assert abs(sums[i] - kn) <= bound, (i, n)
         elif k == raja.REQ_EXCLUSIVE_SCAN_I32:
             want = orc.seq_exclusive(sin[o:o + n], orc.PLUS, 0)
@@ -131,8 +131,9 @@ def test_serve_mix_matches_oracle_and_modes_agree(raja, orc):
     assert all(r.status == 0 and r.unit_size == 1 for r in res)
     single = pools.snapshot()
     srv.close()
+    # slots of other request kinds stay NaN in the sums array
     for a, b in zip(batched, single):
-        assert np.array_equal(a, b, equal_nan=False)
+        assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
 
     # a sum served in a batch has the bits of the span reduction of its range
     for i, (k, (o, n)) in enumerate(zip(kinds, pools.spans)):
