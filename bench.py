@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--no-ops", action="store_true", help="skip the secondary ops")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving stream")
     ap.add_argument("--no-halo", action="store_true", help="skip the WorkGroup halo exchange")
+    ap.add_argument("--no-forall", action="store_true", help="skip daxpy + dot + MinLoc")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -148,6 +149,14 @@ def main():
         del keys0, vals0
         ops = secondary_ops(R, torch, np, dev, stream, n)
 
+    fa = None
+    if not args.no_forall:
+        fa = run_example("daxpy_dot_minloc", ["24", "20"])
+        if fa:
+            for k in ("daxpy", "dot_minloc_reducers", "dot_minloc_params"):
+                log("bench: forall %-20s warm %.4f ms (%.1f GB/s)  cold %.4f ms (%.1f GB/s)"
+                    % (k, fa[k]["warm_ms"], fa[k]["warm_GBs"], fa[k]["cold_ms"], fa[k]["cold_GBs"]))
+
     halo = None
     if not args.no_halo:
         halo = halo_exchange()
@@ -195,6 +204,7 @@ def main():
         "ops": ops,
         "serving": serve,
         "workgroup_halo": halo,
+        "forall_daxpy_dot_minloc": fa,
     }
     print(json.dumps(line), flush=True)
 
@@ -321,20 +331,27 @@ def segmented_ops(R, torch, np, dev, stream, rec):
     return out
 
 
+def run_example(name, args):
+    """Runs raja_amd/bin/<name> (a C++ front-end program) in a child process
+    and returns the JSON line it prints."""
+    import subprocess
+    exe = os.path.join(ROOT, "raja_amd", "bin", name)
+    if not os.path.exists(exe):
+        log("bench: raja_amd/bin/%s not built; skipped" % name)
+        return None
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("%s failed: %s%s" % (name, r.stdout, r.stderr))
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def halo_exchange():
     """The WorkGroup halo exchange (256^3 grid, halo 2, 10 FP64 variables,
     260 loops per phase) run by raja_amd/bin/halo_exchange in a child
     process: per-loop forall, ordered WorkGroup, unordered tile WorkGroup."""
-    import subprocess
-    exe = os.path.join(ROOT, "raja_amd", "bin", "halo_exchange")
-    if not os.path.exists(exe):
-        log("bench: raja_amd/bin/halo_exchange not built; halo skipped")
+    out = run_example("halo_exchange", ["256", "256", "256", "2", "10", "5"])
+    if out is None:
         return None
-    r = subprocess.run([exe, "256", "256", "256", "2", "10", "5"], capture_output=True, text=True,
-                       timeout=300)
-    if r.returncode != 0:
-        raise RuntimeError("halo_exchange failed: " + r.stdout + r.stderr)
-    out = json.loads(r.stdout.strip().splitlines()[-1])
     ph = lambda v, k: out[v]["pack_%s_ms" % k] + out[v]["unpack_%s_ms" % k]
     out["speedup_unordered_vs_ordered_device"] = round(ph("ordered", "device") / ph("unordered", "device"), 3)
     out["speedup_unordered_vs_forall_host"] = round(ph("forall", "host") / ph("unordered", "host"), 3)

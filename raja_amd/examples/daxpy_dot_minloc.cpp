@@ -1,0 +1,217 @@
+// daxpy, then dot + MinLoc, through RAJA::forall on the cdna4 policies --
+// the forall/reducer workload of the reference's docs and tests
+// (docs/sphinx/user_guide/feature/reduction.rst:280-323,
+// include/RAJA/policy/hip/forall.hpp:553-639, include/RAJA/policy/hip/reduce.hpp:960-1161):
+//   y = 2.5 * x + y                                   (24 B per element)
+//   dot = sum x[i]*y[i], (min, argmin) of x           (16 B per element, one pass)
+// The second loop is run twice: with old-style reducers (ReduceSum,
+// ReduceMinLoc: get() synchronises) and with expt::Reduce parameters
+// (resolved at the end of forall).  x, y ~ U[-1, 1) from the library's
+// counter generator (seed 0 and 7).
+//
+// Each loop is timed warm (inputs left in the 256 MiB last-level cache when
+// they fit) and cold (a 512 MiB scratch buffer written before every
+// repetition).  Prints one JSON line.
+//
+// Usage: daxpy_dot_minloc [log2n] [reps]     (default 24 20)
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <vector>
+
+#include "RAJA/RAJA.hpp"
+#include "raja_cdna4.h"
+
+namespace
+{
+
+void check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess)
+  {
+    std::fprintf(stderr, "daxpy_dot_minloc: %s: %s\n", what, hipGetErrorString(e));
+    std::exit(2);
+  }
+}
+
+void check_lib(int st, const char* what)
+{
+  if (st != RAJA_CDNA4_SUCCESS)
+  {
+    std::fprintf(stderr, "daxpy_dot_minloc: %s: %s\n", what, raja_cdna4_last_error());
+    std::exit(2);
+  }
+}
+
+double median(std::vector<double> v)
+{
+  std::sort(v.begin(), v.end());
+  return v[v.size() / 2];
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+  const int log2n = argc > 1 ? std::atoi(argv[1]) : 24;
+  const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
+  if (log2n < 4 || log2n > 30 || reps < 1)
+  {
+    std::fprintf(stderr, "usage: %s [log2n 4..30] [reps >= 1]\n", argv[0]);
+    return 1;
+  }
+  const RAJA::Index_type N = (RAJA::Index_type)1 << log2n;
+  check_lib(raja_cdna4_init(0), "init");
+  auto res = RAJA::resources::Cdna4::get_default();
+  hipStream_t st = res.get_stream();
+  double *x = nullptr, *y = nullptr, *y0 = nullptr;
+  check(hipMalloc(&x, sizeof(double) * N), "hipMalloc");
+  check(hipMalloc(&y, sizeof(double) * N), "hipMalloc");
+  check(hipMalloc(&y0, sizeof(double) * N), "hipMalloc");
+  const size_t flush_bytes = (size_t)512 << 20;
+  uint32_t* flush = nullptr;
+  check(hipMalloc(&flush, flush_bytes), "hipMalloc");
+  // fill kinds of raja_cdna4_fill: 3 = FP64 uniform [a, b)
+  check_lib(raja_cdna4_fill(3, x, N, 0, -1.0, 1.0, st), "fill");
+  check_lib(raja_cdna4_fill(3, y0, N, 7, -1.0, 1.0, st), "fill");
+  check(hipStreamSynchronize(st), "sync");
+
+  using EXEC = RAJA::cdna4_exec<256>;
+  using RED = RAJA::cdna4_reduce;
+  const double a = 2.5;
+  auto cold = [&]() {
+    // 7 = uint32 constant fill: overwrites the last-level cache
+    check_lib(raja_cdna4_fill(7, flush, flush_bytes / 4, 0, 1.0, 0.0, st), "flush");
+  };
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "event");
+  check(hipEventCreate(&e1), "event");
+
+  struct Row
+  {
+    std::vector<double> warm, cold, host;
+  } daxpy, old_style, params;
+  double dot_old = 0, dot_par = 0, min_old = 0, min_par = 0;
+  RAJA::Index_type loc_old = -1, loc_par = -1;
+
+  for (int r = 0; r < 2 * reps + 1; ++r)
+  {
+    const bool is_cold = (r % 2) == 1;
+    const bool keep = r > 0;
+    // daxpy on a fresh copy of y
+    check(hipMemcpyAsync(y, y0, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+    if (is_cold) cold();
+    check(hipEventRecord(e0, st), "record");
+    RAJA::forall<EXEC>(res, RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) { y[i] = a * x[i] + y[i]; });
+    check(hipEventRecord(e1, st), "record");
+    check(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (keep) (is_cold ? daxpy.cold : daxpy.warm).push_back(ms);
+
+    // dot + MinLoc with old-style reducers
+    if (is_cold) cold();
+    check(hipStreamSynchronize(st), "sync");
+    {
+      const auto h0 = std::chrono::steady_clock::now();
+      RAJA::ReduceSum<RED, double> dot(0.0);
+      RAJA::ReduceMinLoc<RED, double> mloc;
+      check(hipEventRecord(e0, st), "record");
+      RAJA::forall<EXEC>(res, RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+        dot += x[i] * y[i];
+        mloc.minloc(x[i], i);
+      });
+      check(hipEventRecord(e1, st), "record");
+      dot_old = dot.get();
+      min_old = mloc.get();
+      loc_old = mloc.getLoc();
+      const double hs = std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
+      check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      if (keep)
+      {
+        (is_cold ? old_style.cold : old_style.warm).push_back(ms);
+        old_style.host.push_back(1e3 * hs);
+      }
+    }
+
+    // the same with expt::Reduce parameters
+    if (is_cold) cold();
+    check(hipStreamSynchronize(st), "sync");
+    {
+      const auto h0 = std::chrono::steady_clock::now();
+      double dot = 0.0;
+      RAJA::expt::ValLoc<double> ml(std::numeric_limits<double>::max(), -1);
+      using VSUM = RAJA::expt::ValOp<double, RAJA::operators::plus>;
+      using VLMIN = RAJA::expt::ValLocOp<double, RAJA::Index_type, RAJA::operators::minimum>;
+      check(hipEventRecord(e0, st), "record");
+      RAJA::forall<EXEC>(res, RAJA::RangeSegment(0, N), RAJA::expt::Reduce<RAJA::operators::plus>(&dot),
+                         RAJA::expt::Reduce<RAJA::operators::minimum>(&ml),
+                         [=] RAJA_DEVICE(RAJA::Index_type i, VSUM& s, VLMIN& l) {
+                           s += x[i] * y[i];
+                           l.minloc(x[i], i);
+                         });
+      check(hipEventRecord(e1, st), "record");
+      check(hipEventSynchronize(e1), "sync");
+      const double hs = std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
+      dot_par = dot;
+      min_par = ml.getVal();
+      loc_par = ml.getLoc();
+      check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      if (keep)
+      {
+        (is_cold ? params.cold : params.warm).push_back(ms);
+        params.host.push_back(1e3 * hs);
+      }
+    }
+  }
+
+  // Host check of the last repetition: MinLoc exact (first index of the
+  // minimum), dot within 2*ceil(log2 N)*eps*sum|x*y| of a compensated sum.
+  std::vector<double> hx(N), hy(N), hy0(N);
+  check(hipMemcpy(hx.data(), x, sizeof(double) * N, hipMemcpyDeviceToHost), "copy back");
+  check(hipMemcpy(hy.data(), y, sizeof(double) * N, hipMemcpyDeviceToHost), "copy back");
+  check(hipMemcpy(hy0.data(), y0, sizeof(double) * N, hipMemcpyDeviceToHost), "copy back");
+  bool daxpy_exact = true;
+  double s = 0.0, c = 0.0, ab = 0.0;
+  RAJA::Index_type im = 0;
+  for (RAJA::Index_type i = 0; i < N; ++i)
+  {
+    const double yi = a * hx[i] + hy0[i];
+    daxpy_exact = daxpy_exact && (yi == hy[i]);
+    const double p = hx[i] * hy[i];
+    const double t = s + p;
+    c += (std::fabs(s) >= std::fabs(p)) ? (s - t) + p : (p - t) + s;
+    s = t;
+    ab += std::fabs(p);
+    if (hx[i] < hx[im]) im = i;
+  }
+  const double kn = s + c;
+  const double bound = 2.0 * log2n * std::ldexp(1.0, -53) * ab;
+  const bool ok = daxpy_exact && std::fabs(dot_old - kn) <= bound && std::fabs(dot_par - kn) <= bound &&
+                  dot_old == dot_par && min_old == hx[im] && loc_old == im && min_par == hx[im] &&
+                  loc_par == im;
+  const double bd = 24.0 * N, br = 16.0 * N;
+  auto gbs = [](double bytes, double ms) { return bytes / (ms * 1e-3) / 1e9; };
+  std::printf("{\"n\": %lld, \"reps\": %d, \"pass\": %s, \"daxpy_bit_exact\": %s, \"dot\": %.17g, "
+              "\"dot_abs_err_vs_kn\": %.3g, \"dot_bound\": %.3g, \"minloc\": [%.17g, %lld], "
+              "\"daxpy\": {\"warm_ms\": %.4f, \"cold_ms\": %.4f, \"warm_GBs\": %.1f, \"cold_GBs\": %.1f}, "
+              "\"dot_minloc_reducers\": {\"warm_ms\": %.4f, \"cold_ms\": %.4f, \"warm_GBs\": %.1f, "
+              "\"cold_GBs\": %.1f, \"host_ms_incl_get\": %.4f}, "
+              "\"dot_minloc_params\": {\"warm_ms\": %.4f, \"cold_ms\": %.4f, \"warm_GBs\": %.1f, "
+              "\"cold_GBs\": %.1f, \"host_ms_incl_resolve\": %.4f}}\n",
+              (long long)N, reps, ok ? "true" : "false", daxpy_exact ? "true" : "false", dot_old,
+              std::fabs(dot_old - kn), bound, min_old, (long long)loc_old, median(daxpy.warm),
+              median(daxpy.cold), gbs(bd, median(daxpy.warm)), gbs(bd, median(daxpy.cold)),
+              median(old_style.warm), median(old_style.cold), gbs(br, median(old_style.warm)),
+              gbs(br, median(old_style.cold)), median(old_style.host), median(params.warm),
+              median(params.cold), gbs(br, median(params.warm)), gbs(br, median(params.cold)),
+              median(params.host));
+  check(hipFree(x), "free");
+  check(hipFree(y), "free");
+  check(hipFree(y0), "free");
+  check(hipFree(flush), "free");
+  return ok ? 0 : 3;
+}
