@@ -17,6 +17,7 @@
 #include "RAJA/pattern/reduce.hpp"
 #include "RAJA/policy/host.hpp"
 #include "RAJA/pattern/WorkGroup.hpp"
+#include "RAJA/pattern/launch.hpp"
 #if defined(RAJA_ENABLE_CDNA4)
 #include "RAJA/policy/cdna4.hpp"
 #endif

@@ -55,6 +55,15 @@ public:
   RAJA_HOST_DEVICE iterator begin() const { return m_data; }
   RAJA_HOST_DEVICE iterator end() const { return m_data + m_size; }
   RAJA_HOST_DEVICE const StorageT* data() const { return m_data; }
+  // Non-owning view of entries [begin, begin + length), clipped to the end.
+  RAJA_HOST_DEVICE TypedListSegment slice(Index_type begin, Index_type length) const
+  {
+    TypedListSegment s(*this);
+    const Index_type b = begin < 0 ? 0 : (begin > m_size ? m_size : begin);
+    s.m_data = m_data + b;
+    s.m_size = (length > m_size - b) ? m_size - b : (length < 0 ? 0 : length);
+    return s;
+  }
 
 private:
   static void free_host(StorageT* p, void*) { std::free(p); }

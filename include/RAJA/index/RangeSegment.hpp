@@ -39,7 +39,18 @@ struct TypedRangeSegment
   RAJA_HOST_DEVICE constexpr iterator end() const { return iterator{m_end}; }
   RAJA_HOST_DEVICE constexpr DiffT size() const { return DiffT(m_end) - DiffT(m_begin); }
   RAJA_HOST_DEVICE constexpr StorageT at(DiffT k) const { return m_begin + StorageT(k); }
+  // Elements [begin, begin + length) of this range, clipped to its end.
+  RAJA_HOST_DEVICE constexpr TypedRangeSegment slice(DiffT begin, DiffT length) const
+  {
+    return TypedRangeSegment(clip(begin), clip(begin + length));
+  }
   static constexpr bool contiguous = true;
+
+private:
+  RAJA_HOST_DEVICE constexpr StorageT clip(DiffT k) const
+  {
+    return (k <= 0) ? m_begin : (k >= size() ? m_end : m_begin + StorageT(k));
+  }
 };
 
 template<typename StorageT>
@@ -68,6 +79,16 @@ struct TypedRangeStrideSegment
   RAJA_HOST_DEVICE constexpr iterator begin() const { return iterator{m_begin, m_stride}; }
   RAJA_HOST_DEVICE constexpr DiffT size() const { return m_size; }
   RAJA_HOST_DEVICE constexpr StorageT at(DiffT k) const { return m_begin + StorageT(k) * m_stride; }
+  // Elements [begin, begin + length) of this range, clipped to its end.
+  RAJA_HOST_DEVICE TypedRangeStrideSegment slice(DiffT begin, DiffT length) const
+  {
+    TypedRangeStrideSegment s(*this);
+    const DiffT b = begin < 0 ? 0 : (begin > m_size ? m_size : begin);
+    const DiffT e = (length > m_size - b) ? m_size : b + (length < 0 ? 0 : length);
+    s.m_begin = m_begin + StorageT(b) * m_stride;
+    s.m_size = e - b;
+    return s;
+  }
   static constexpr bool contiguous = false;
 };
 

@@ -4,6 +4,7 @@
 #include "RAJA/policy/cdna4/forall.hpp"
 #include "RAJA/policy/cdna4/algorithms.hpp"
 #include "RAJA/policy/cdna4/WorkGroup.hpp"
+#include "RAJA/policy/cdna4/launch.hpp"
 
 namespace RAJA
 {

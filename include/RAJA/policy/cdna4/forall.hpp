@@ -100,16 +100,52 @@ inline LaunchInfo& launch_info()
 template<typename Op>
 using tree_of = reduce::detail::tree_of<Op>;
 
-// Block fold at kernel end; thread 0 stores the block value.  Every thread
-// of the block must call it (it synchronises the block).
+// Block fold at kernel end; the block's first thread stores the block value
+// at the block's flat index.  Every thread of the block must call it (it
+// synchronises the block).  Threads and blocks are numbered x fastest, then
+// y, then z.  Full waves fold with the DPP wave tree; a partial last wave
+// (block sizes that are not a multiple of 64) folds its lanes in order
+// through LDS; the wave values are folded in wave order.
 template<typename T, typename TreeOp>
 __device__ inline void store_block_value(T v, T* slots)
 {
+  using namespace RAJA::cdna4::dev;
   static_assert(sizeof(T) % 4 == 0, "cdna4 reducers need 4-byte multiple types");
   // raw LDS bytes: __shared__ objects cannot have constructors
-  __shared__ __attribute__((aligned(16))) unsigned char s_raw[sizeof(T) * RAJA::cdna4::dev::max_waves];
-  const T r = RAJA::cdna4::dev::block_reduce(v, TreeOp{}, reinterpret_cast<T*>(s_raw));
-  if (threadIdx.x == 0) slots[blockIdx.x] = r;
+  __shared__ __attribute__((aligned(16))) unsigned char s_raw[sizeof(T) * max_waves];
+  __shared__ __attribute__((aligned(16))) unsigned char s_part[sizeof(T) * wave_size];
+  T* s_wave = reinterpret_cast<T*>(s_raw);
+  T* s_lane = reinterpret_cast<T*>(s_part);
+  const TreeOp op{};
+  const int nthreads = (int)(blockDim.x * blockDim.y * blockDim.z);
+  const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+  const int w = t / wave_size;
+  const int nw = (nthreads + wave_size - 1) / wave_size;
+  const int last = nthreads - (nw - 1) * wave_size;  // lanes in the last wave
+  if (w < nw - 1 || last == wave_size)
+  {
+    const T r = wave_reduce(v, op);
+    if (lane_id() == 0) s_wave[w] = r;
+  }
+  else
+  {
+    s_lane[t - w * wave_size] = v;
+  }
+  __syncthreads();
+  if (t == 0)
+  {
+    if (last != wave_size)
+    {
+      T r = s_lane[0];
+      for (int k = 1; k < last; ++k) r = op(r, s_lane[k]);
+      s_wave[nw - 1] = r;
+    }
+    T r = s_wave[0];
+    for (int k = 1; k < nw; ++k) r = op(r, s_wave[k]);
+    const int b = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    slots[b] = r;
+  }
+  __syncthreads();
 }
 
 inline void check(int status, const char* what)

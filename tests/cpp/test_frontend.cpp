@@ -4,6 +4,7 @@
 // (test/functional/forall/reduce-basic, test/functional/scan,
 // test/unit/algorithm, test/functional/workgroup, test/unit/resource).
 // Prints one line per check and exits with the number of failures.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -213,6 +214,174 @@ void test_host_workgroup(const char* tag)
 }
 
 // ---- cdna4 --------------------------------------------------------------------------
+
+// ---- RAJA::launch cases (test/functional/launch in the reference) ---------------
+// One routine runs on the host (ExecPlace::HOST, seq policies, host
+// pointers) and on the device (ExecPlace::DEVICE, cdna4 policies, device
+// pointers); the results are compared with closed forms on the host.
+
+struct LaunchResults
+{
+  long long sum = 0, sum2 = 2;
+  RAJA::expt::ValLoc<double> mloc{std::numeric_limits<double>::max(), -1};
+  long long msum = 0;
+  long long empty_sum = 5;
+};
+
+template<typename LPOL, typename GLOBAL, typename BLOCKS, typename TX, typename TY>
+LaunchResults run_launch_cases(RAJA::ExecPlace place, int N, const int* a, const double* x,
+                               int R, int D, int C, const int* m, int* m2, double* team_out,
+                               double* y, long long* z, int nteams1d)
+{
+  LaunchResults out;
+  using VSUM = RAJA::expt::ValOp<long long, RAJA::operators::plus>;
+  using VLMIN = RAJA::expt::ValLocOp<double, RAJA::Index_type, RAJA::operators::minimum>;
+  // (a) global-thread loop with three parameter reducers
+  RAJA::launch<LPOL>(place, RAJA::LaunchParams(RAJA::Teams(nteams1d), RAJA::Threads(256)),
+                     RAJA::expt::Reduce<RAJA::operators::plus>(&out.sum),
+                     RAJA::expt::Reduce<RAJA::operators::plus>(&out.sum2),
+                     RAJA::expt::Reduce<RAJA::operators::minimum>(&out.mloc), RAJA::Name("launch-a"),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx, VSUM& s, VSUM& s2, VLMIN& ml) {
+                       RAJA::loop<GLOBAL>(ctx, RAJA::RangeSegment(0, N), [&](RAJA::Index_type i) {
+                         s += a[i];
+                         s2 += a[i];
+                         ml.minloc(x[i], i);
+                       });
+                     });
+  // (b) three nested loops over a 70-thread (10 x 7) team, with a reducer
+  RAJA::launch<LPOL>(place, RAJA::LaunchParams(RAJA::Teams(16), RAJA::Threads(10, 7)),
+                     RAJA::expt::Reduce<RAJA::operators::plus>(&out.msum),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx, VSUM& s) {
+                       RAJA::loop<BLOCKS>(ctx, RAJA::RangeSegment(0, R), [&](RAJA::Index_type r) {
+                         RAJA::loop<TY>(ctx, RAJA::RangeSegment(0, D), [&](RAJA::Index_type d) {
+                           RAJA::loop<TX>(ctx, RAJA::RangeSegment(0, C), [&](RAJA::Index_type c) {
+                             const RAJA::Index_type k = (r * D + d) * C + c;
+                             m2[k] = 2 * m[k];
+                             s += m[k];
+                           });
+                         });
+                       });
+                     });
+  // (c) team shared memory and teamSync: team t sums x[128 t .. 128 t + 127]
+  const int T = (N + 127) / 128;
+  RAJA::launch<LPOL>(place, RAJA::LaunchParams(RAJA::Teams(T), RAJA::Threads(128), 128 * sizeof(double)),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx) {
+                       RAJA::loop<BLOCKS>(ctx, RAJA::RangeSegment(0, T), [&](RAJA::Index_type t) {
+                         double* sh = ctx.getSharedMemory<double>(128);
+                         RAJA::loop<TX>(ctx, RAJA::RangeSegment(0, 128), [&](RAJA::Index_type k) {
+                           const RAJA::Index_type i = t * 128 + k;
+                           sh[k] = (i < N) ? x[i] : 0.0;
+                         });
+                         ctx.teamSync();
+                         RAJA::loop<TX>(ctx, RAJA::RangeSegment(0, 1), [&](RAJA::Index_type) {
+                           double acc = 0.0;
+                           for (int k = 0; k < 128; ++k) acc += sh[k];
+                           team_out[t] = acc;
+                         });
+                         ctx.teamSync();
+                         ctx.releaseSharedMemory();
+                       });
+                     });
+  // (d) tile + loop_icount, tile_tcount
+  RAJA::launch<LPOL>(place, RAJA::LaunchParams(RAJA::Teams(64), RAJA::Threads(256)),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx) {
+                       RAJA::tile<BLOCKS>(ctx, 256, RAJA::RangeSegment(0, N), [&](RAJA::RangeSegment const& ts) {
+                         RAJA::loop_icount<TX>(ctx, ts, [&](RAJA::Index_type i, RAJA::Index_type k) {
+                           y[i] = x[i] + (double)k;
+                         });
+                       });
+                       RAJA::tile_tcount<BLOCKS>(ctx, 256, RAJA::RangeSegment(0, N),
+                                                 [&](RAJA::RangeSegment const& ts, RAJA::Index_type tix) {
+                                                   RAJA::loop<TX>(ctx, ts, [&](RAJA::Index_type i) { z[i] = tix; });
+                                                 });
+                     });
+  // (e) no teams: the parameter keeps its target value
+  RAJA::launch<LPOL>(place, RAJA::LaunchParams(RAJA::Teams(0), RAJA::Threads(64)),
+                     RAJA::expt::Reduce<RAJA::operators::plus>(&out.empty_sum),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx, VSUM& s) {
+                       RAJA::loop<GLOBAL>(ctx, RAJA::RangeSegment(0, N), [&](RAJA::Index_type i) { s += a[i]; });
+                     });
+  return out;
+}
+
+void check_launch_results(const std::string& t, bool device, const LaunchResults& got, int N,
+                          const std::vector<int>& a,
+                          const std::vector<double>& x, const std::vector<int>& m,
+                          const std::vector<int>& m2, const std::vector<double>& team_out,
+                          const std::vector<double>& y, const std::vector<long long>& z)
+{
+  long long ref = 0;
+  for (int v : a) ref += v;
+  const int im = (int)(std::min_element(x.begin(), x.end()) - x.begin());
+  long long mref = 0;
+  bool m2ok = true;
+  for (size_t k = 0; k < m.size(); ++k)
+  {
+    mref += m[k];
+    m2ok = m2ok && m2[k] == 2 * m[k];
+  }
+  bool tok = true;
+  for (size_t t2 = 0; t2 < team_out.size(); ++t2)
+  {
+    double acc = 0.0;
+    for (int k = 0; k < 128; ++k)
+    {
+      const size_t i = t2 * 128 + k;
+      acc += (i < (size_t)N) ? x[i] : 0.0;
+    }
+    tok = tok && acc == team_out[t2];
+  }
+  bool yok = true, zok = true;
+  for (int i = 0; i < N; ++i)
+  {
+    yok = yok && y[i] == x[i] + (double)(i % 256);
+    zok = zok && z[i] == i / 256;
+  }
+  CHECK(got.sum == ref, (t + " launch expt sum").c_str());
+  CHECK(got.sum2 == ref + 2, (t + " launch expt sum init offset").c_str());
+  CHECK(got.mloc.getVal() == x[im] && got.mloc.getLoc() == im, (t + " launch expt minloc").c_str());
+  CHECK(got.msum == mref && m2ok, (t + " launch nested loops, 70-thread teams").c_str());
+  CHECK(tok, (t + " launch shared memory + teamSync").c_str());
+  CHECK(yok, (t + " launch tile + loop_icount").c_str());
+  CHECK(zok, (t + " launch tile_tcount").c_str());
+  // a device launch with no teams runs nothing; the sequential launch runs
+  // its body once whatever the team count (include/RAJA/policy/sequential/launch.hpp)
+  CHECK(got.empty_sum == (device ? 5 : 5 + ref), (t + " launch with no teams").c_str());
+}
+
+struct LaunchData
+{
+  int N = 100003, R = 50, D = 13, C = 77;
+  std::vector<int> a, m;
+  std::vector<double> x;
+  LaunchData()
+  {
+    a.resize(N);
+    x.resize(N);
+    m.resize((size_t)R * D * C);
+    for (int i = 0; i < N; ++i)
+    {
+      a[i] = (int)(mix(11, i) % 1000) - 500;
+      x[i] = (double)((int)(mix(12, i) % 2001) - 1000) * 0.5;  // ties: first index wins
+    }
+    for (size_t k = 0; k < m.size(); ++k) m[k] = (int)(mix(13, k) % 100);
+  }
+};
+
+void test_host_launch()
+{
+  LaunchData L;
+  using LP = RAJA::LaunchPolicy<RAJA::seq_launch_t>;
+  using S = RAJA::LoopPolicy<RAJA::seq_exec>;
+  std::vector<int> m2(L.m.size(), 0);
+  const int T = (L.N + 127) / 128;
+  std::vector<double> team_out(T), y(L.N);
+  std::vector<long long> z(L.N);
+  LaunchResults r = run_launch_cases<LP, S, S, S, S>(RAJA::ExecPlace::HOST, L.N, L.a.data(), L.x.data(), L.R,
+                                                     L.D, L.C, L.m.data(), m2.data(), team_out.data(),
+                                                     y.data(), z.data(), (L.N + 255) / 256);
+  check_launch_results("seq", false, r, L.N, L.a, L.x, L.m, m2, team_out, y, z);
+}
 
 #if defined(RAJA_ENABLE_CDNA4)
 
@@ -575,6 +744,78 @@ void test_cdna4_workgroups()
   dfree(dbuf);
 }
 
+void test_cdna4_launch()
+{
+  LaunchData L;
+  using LP = RAJA::LaunchPolicy<RAJA::seq_launch_t, RAJA::cdna4_launch_t<false>>;
+  using G = RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_global_thread_x>;
+  using B = RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_block_x_loop>;
+  using TX = RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_thread_x_loop>;
+  using TY = RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_thread_y_loop>;
+  const int T = (L.N + 127) / 128;
+  int* da = to_device(L.a);
+  double* dx = to_device(L.x);
+  int* dm = to_device(L.m);
+  int* dm2 = to_device(std::vector<int>(L.m.size(), 0));
+  double* dteam = to_device(std::vector<double>(T, 0.0));
+  double* dy = to_device(std::vector<double>(L.N, 0.0));
+  long long* dz = to_device(std::vector<long long>(L.N, 0));
+  LaunchResults r = run_launch_cases<LP, G, B, TX, TY>(RAJA::ExecPlace::DEVICE, L.N, da, dx, L.R, L.D, L.C, dm,
+                                                       dm2, dteam, dy, dz, (L.N + 255) / 256);
+  check_launch_results("cdna4", true, r, L.N, L.a, L.x, L.m, to_host(dm2, L.m.size()), to_host(dteam, T),
+                       to_host(dy, L.N), to_host(dz, L.N));
+  // the same policy list on the host place runs the seq entries on host data
+  {
+    std::vector<int> m2(L.m.size(), 0);
+    std::vector<double> team_out(T), y(L.N);
+    std::vector<long long> z(L.N);
+    LaunchResults h = run_launch_cases<LP, G, B, TX, TY>(RAJA::ExecPlace::HOST, L.N, L.a.data(), L.x.data(),
+                                                         L.R, L.D, L.C, L.m.data(), m2.data(),
+                                                         team_out.data(), y.data(), z.data(),
+                                                         (L.N + 255) / 256);
+    check_launch_results("cdna4-list host place", false, h, L.N, L.a, L.x, L.m, m2, team_out, y, z);
+  }
+  // determinism: the same launch shape gives the same bits
+  const int N = L.N;
+  double s1 = 0.0, s2 = 0.0;
+  using VD = RAJA::expt::ValOp<double, RAJA::operators::plus>;
+  for (double* tgt : {&s1, &s2})
+  {
+    RAJA::launch<LP>(RAJA::ExecPlace::DEVICE, RAJA::LaunchParams(RAJA::Teams(97), RAJA::Threads(96)),
+                     RAJA::expt::Reduce<RAJA::operators::plus>(tgt),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx, VD& s) {
+                       RAJA::loop<RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_global_x_loop>>(
+                           ctx, RAJA::RangeSegment(0, N), [&](RAJA::Index_type i) { s += dx[i] * 1e-3; });
+                     });
+  }
+  CHECK(s1 == s2, "cdna4 launch FP sum deterministic (96-thread teams)");
+  // old-style reducers captured by a launch body
+  {
+    RAJA::ReduceSum<RAJA::cdna4_reduce, long long> sum(0);
+    RAJA::ReduceMinLoc<RAJA::cdna4_reduce, double> ml;
+    RAJA::launch<LP>(RAJA::ExecPlace::DEVICE, RAJA::LaunchParams(RAJA::Teams(200), RAJA::Threads(100)),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx) {
+                       RAJA::loop<RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_global_x_loop>>(
+                           ctx, RAJA::RangeSegment(0, N), [&](RAJA::Index_type i) {
+                             sum += da[i];
+                             ml.minloc(dx[i], i);
+                           });
+                     });
+    long long ref = 0;
+    for (int v : L.a) ref += v;
+    const int im = (int)(std::min_element(L.x.begin(), L.x.end()) - L.x.begin());
+    CHECK(sum.get() == ref, "cdna4 launch old-style ReduceSum");
+    CHECK(ml.get() == L.x[im] && ml.getLoc() == im, "cdna4 launch old-style ReduceMinLoc");
+  }
+  dfree(da);
+  dfree(dx);
+  dfree(dm);
+  dfree(dm2);
+  dfree(dteam);
+  dfree(dy);
+  dfree(dz);
+}
+
 #endif
 
 int main()
@@ -587,12 +828,14 @@ int main()
   test_host_scans_sorts<RAJA::omp_parallel_for_exec>("omp");
   test_host_workgroup<RAJA::seq_work>("seq");
   test_host_workgroup<RAJA::omp_work>("omp");
+  test_host_launch();
 #if defined(RAJA_ENABLE_CDNA4)
   test_cdna4_forall_reducers();
   test_cdna4_daxpy_and_params();
   test_cdna4_segments_resources();
   test_cdna4_scans_sorts();
   test_cdna4_workgroups();
+  test_cdna4_launch();
 #endif
   std::printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
