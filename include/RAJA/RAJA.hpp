@@ -1,5 +1,6 @@
-// Umbrella header of the cdna4 RAJA front-end subset: forall, reducers
-// (old style and expt::Reduce parameters), scans, sorts, WorkGroup,
+// Umbrella header of the cdna4 RAJA front-end subset: forall, launch,
+// reducers (old style, multi (binned) and expt::Reduce parameters), scans,
+// sorts, WorkGroup,
 // resources and synchronize, for the seq_exec, omp_parallel_for_exec and
 // cdna4 (MI355X) policy families.  The cdna4 family is enabled when the
 // translation unit is compiled by hipcc.  Backends are included before the
@@ -15,6 +16,7 @@
 #include "RAJA/policy/PolicyBase.hpp"
 #include "RAJA/pattern/params.hpp"
 #include "RAJA/pattern/reduce.hpp"
+#include "RAJA/pattern/multi_reduce.hpp"
 #include "RAJA/policy/host.hpp"
 #include "RAJA/pattern/WorkGroup.hpp"
 #include "RAJA/pattern/launch.hpp"

@@ -5,6 +5,7 @@
 #include "RAJA/policy/cdna4/algorithms.hpp"
 #include "RAJA/policy/cdna4/WorkGroup.hpp"
 #include "RAJA/policy/cdna4/launch.hpp"
+#include "RAJA/policy/cdna4/multi_reduce.hpp"
 
 namespace RAJA
 {

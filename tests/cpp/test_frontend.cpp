@@ -215,6 +215,89 @@ void test_host_workgroup(const char* tag)
 
 // ---- cdna4 --------------------------------------------------------------------------
 
+// ---- multi (binned) reducers (test/functional/forall/multi-reduce-basic) ----------
+
+template<typename EXEC, typename MPOL, typename A, typename D>
+void run_multi_cases(const std::string& t, int N, int nb, A a, D x, const std::vector<int>& ha,
+                     const std::vector<double>& hx)
+{
+  std::vector<long long> init(nb);
+  for (int b = 0; b < nb; ++b) init[b] = b;
+  RAJA::MultiReduceSum<MPOL, long long> isum(init);
+  RAJA::MultiReduceSum<MPOL, double> dsum(nb);
+  RAJA::MultiReduceMin<MPOL, double> dmin(nb);
+  RAJA::MultiReduceMax<MPOL, int> imax(nb, -5000);
+  RAJA::MultiReduceBitOr<MPOL, int> bor(nb);
+  RAJA::MultiReduceBitAnd<MPOL, int> band(nb);
+  for (int rep = 0; rep < 2; ++rep)  // results accumulate across loops
+  {
+    RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=] RAJA_HOST_DEVICE(RAJA::Index_type i) {
+      const int b = (int)(i % nb);
+      isum[b] += a[i];
+      dsum[b] += x[i];
+      dmin[b].min(x[i]);
+      imax[b].max(a[i]);
+      bor[b] |= (a[i] & 0xFF);
+      band[b] &= (a[i] | 0x100);
+    });
+  }
+  std::vector<long long> ris(nb);
+  std::vector<double> rds(nb), rmin(nb, std::numeric_limits<double>::max());
+  std::vector<int> rmax(nb, -5000), ror(nb, 0), rand_(nb, ~0);
+  std::vector<double> abs_sum(nb, 0.0);
+  for (int b = 0; b < nb; ++b) ris[b] = b;
+  for (int rep = 0; rep < 2; ++rep)
+  {
+    for (int i = 0; i < N; ++i)
+    {
+      const int b = i % nb;
+      ris[b] += ha[i];
+      rds[b] += hx[i];
+      abs_sum[b] += std::fabs(hx[i]);
+      rmin[b] = std::min(rmin[b], hx[i]);
+      rmax[b] = std::max(rmax[b], ha[i]);
+      ror[b] |= (ha[i] & 0xFF);
+      rand_[b] &= (ha[i] | 0x100);
+    }
+  }
+  bool ok_is = true, ok_ds = true, ok_min = true, ok_max = true, ok_bits = true;
+  for (int b = 0; b < nb; ++b)
+  {
+    ok_is = ok_is && isum.get(b) == ris[b];
+    ok_ds = ok_ds && std::fabs(dsum.get(b) - rds[b]) <= 64 * 2.2e-16 * abs_sum[b] + 1e-300;
+    ok_min = ok_min && dmin.get(b) == rmin[b];
+    ok_max = ok_max && imax.get(b) == rmax[b];
+    ok_bits = ok_bits && bor.get(b) == ror[b] && band.get(b) == rand_[b];
+  }
+  CHECK(ok_is, (t + " MultiReduceSum int64 with init container, two loops").c_str());
+  CHECK(ok_ds, (t + " MultiReduceSum double within bound").c_str());
+  CHECK(ok_min, (t + " MultiReduceMin double").c_str());
+  CHECK(ok_max, (t + " MultiReduceMax int with init").c_str());
+  CHECK(ok_bits, (t + " MultiReduceBitOr / BitAnd").c_str());
+  std::vector<long long> all(nb);
+  isum.get_all(all);
+  CHECK(all == ris && isum.size() == (size_t)nb, (t + " MultiReduce get_all / size").c_str());
+  isum.reset();
+  RAJA::forall<EXEC>(RAJA::RangeSegment(0, N), [=] RAJA_HOST_DEVICE(RAJA::Index_type i) { isum[0] += 1; });
+  CHECK(isum.get(0) == N && isum.get(1) == 0, (t + " MultiReduce reset to identity").c_str());
+  isum.reset(3, 7);
+  CHECK(isum.size() == 3 && isum.get(2) == 7, (t + " MultiReduce reset(num_bins, init)").c_str());
+}
+
+template<typename EXEC, typename MPOL>
+void test_host_multi(const char* tag)
+{
+  const int N = 10007, nb = 37;
+  std::vector<int> a(N);
+  std::vector<double> x(N);
+  for (int i = 0; i < N; ++i)
+  {
+    a[i] = (int)(mix(21, i) % 2000) - 1000;
+    x[i] = (double)((int)(mix(22, i) % 20001) - 10000) * 0.125;
+  }
+  run_multi_cases<EXEC, MPOL>(tag, N, nb, a.data(), x.data(), a, x);
+}
+
 // ---- RAJA::launch cases (test/functional/launch in the reference) ---------------
 // One routine runs on the host (ExecPlace::HOST, seq policies, host
 // pointers) and on the device (ExecPlace::DEVICE, cdna4 policies, device
@@ -816,6 +899,56 @@ void test_cdna4_launch()
   dfree(dz);
 }
 
+void test_cdna4_multi()
+{
+  const int nb = 97;
+  for (int N : {1000, 1 << 20})
+  {
+    std::vector<int> a(N);
+    std::vector<double> x(N);
+    for (int i = 0; i < N; ++i)
+    {
+      a[i] = (int)(mix(21, i) % 2000) - 1000;
+      x[i] = (double)((int)(mix(22, i) % 20001) - 10000) * 0.125;
+    }
+    int* da = to_device(a);
+    double* dx = to_device(x);
+    run_multi_cases<RAJA::cdna4_exec<256>, RAJA::cdna4_multi_reduce>("cdna4 N=" + std::to_string(N), N, nb,
+                                                                      (const int*)da, (const double*)dx, a, x);
+    // inside RAJA::launch, 200 teams of 96 threads
+    RAJA::MultiReduceSum<RAJA::cdna4_multi_reduce, long long> ls(nb);
+    using LP = RAJA::LaunchPolicy<RAJA::seq_launch_t, RAJA::cdna4_launch_t<false>>;
+    RAJA::launch<LP>(RAJA::ExecPlace::DEVICE, RAJA::LaunchParams(RAJA::Teams(200), RAJA::Threads(96)),
+                     [=] RAJA_HOST_DEVICE(RAJA::LaunchContext ctx) {
+                       RAJA::loop<RAJA::LoopPolicy<RAJA::seq_exec, RAJA::cdna4_global_x_loop>>(
+                           ctx, RAJA::RangeSegment(0, N), [&](RAJA::Index_type i) { ls[i % nb] += da[i]; });
+                     });
+    bool ok = true;
+    std::vector<long long> ref(nb, 0);
+    for (int i = 0; i < N; ++i) ref[i % nb] += a[i];
+    for (int b = 0; b < nb; ++b) ok = ok && ls.get(b) == ref[b];
+    CHECK(ok, "cdna4 MultiReduceSum inside launch");
+    dfree(da);
+    dfree(dx);
+  }
+  // many bins: rows shared between blocks (agent-scope atomics), still exact for integers
+  {
+    const int N = 1 << 22, nbig = 1 << 20;
+    std::vector<int> a(N);
+    for (int i = 0; i < N; ++i) a[i] = (int)(mix(23, i) % 100);
+    int* da = to_device(a);
+    RAJA::MultiReduceSum<RAJA::cdna4_multi_reduce, long long> big(nbig);
+    RAJA::forall<RAJA::cdna4_exec<256>>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+      big[(i * 7919) % nbig] += da[i];
+    });
+    std::vector<long long> ref(nbig, 0), got(nbig);
+    for (int i = 0; i < N; ++i) ref[((long long)i * 7919) % nbig] += a[i];
+    big.get_all(got);
+    CHECK(got == ref, "cdna4 MultiReduceSum 2^20 bins (shared rows)");
+    dfree(da);
+  }
+}
+
 #endif
 
 int main()
@@ -829,6 +962,8 @@ int main()
   test_host_workgroup<RAJA::seq_work>("seq");
   test_host_workgroup<RAJA::omp_work>("omp");
   test_host_launch();
+  test_host_multi<RAJA::seq_exec, RAJA::seq_multi_reduce>("seq");
+  test_host_multi<RAJA::omp_parallel_for_exec, RAJA::omp_multi_reduce>("omp");
 #if defined(RAJA_ENABLE_CDNA4)
   test_cdna4_forall_reducers();
   test_cdna4_daxpy_and_params();
@@ -836,6 +971,7 @@ int main()
   test_cdna4_scans_sorts();
   test_cdna4_workgroups();
   test_cdna4_launch();
+  test_cdna4_multi();
 #endif
   std::printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
