@@ -13,6 +13,7 @@
 #include "RAJA/util/resource.hpp"
 #include "RAJA/index/RangeSegment.hpp"
 #include "RAJA/index/ListSegment.hpp"
+#include "RAJA/index/IndexSet.hpp"
 #include "RAJA/policy/PolicyBase.hpp"
 #include "RAJA/pattern/params.hpp"
 #include "RAJA/pattern/reduce.hpp"

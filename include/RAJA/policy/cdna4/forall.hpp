@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "RAJA/config.hpp"
+#include "RAJA/index/IndexSet.hpp"
 #include "RAJA/index/RangeSegment.hpp"
 #include "RAJA/pattern/params.hpp"
 #include "RAJA/pattern/reduce.hpp"
@@ -375,6 +376,24 @@ int occupancy_grid(Kernel k, int block)
 }  // namespace cdna4
 
 RAJA_CDNA4_DECLARE_REDUCERS(cdna4_reduce, cdna4::Cdna4ReduceImpl)
+
+namespace policy
+{
+namespace cdna4
+{
+
+}  // namespace cdna4
+}  // namespace policy
+
+namespace detail
+{
+template<int B, bool A, int G>
+struct segit_exec<policy::cdna4::cdna4_exec_t<B, A, G>>
+{
+  using type = policy::cdna4::cdna4_exec_t<B, true, G>;
+  static constexpr bool wait_after = !A;
+};
+}  // namespace detail
 
 namespace policy
 {

@@ -298,6 +298,53 @@ void test_host_multi(const char* tag)
   run_multi_cases<EXEC, MPOL>(tag, N, nb, a.data(), x.data(), a, x);
 }
 
+// ---- index sets (test/functional/indexset-build, forall/indexset) ---------------
+
+// Range [0, 1000), a list of every 7th index in [1000, 3000), a stride range
+// [3000, 6000) step 5, a range [6000, 6001).
+template<typename IS>
+std::vector<RAJA::Index_type> fill_index_set(IS& iset, const std::vector<RAJA::Index_type>& list_host,
+                                             const RAJA::TypedListSegment<RAJA::Index_type>& list)
+{
+  iset.push_back(RAJA::RangeSegment(0, 1000));
+  iset.push_back(list);
+  iset.push_back(RAJA::RangeStrideSegment(3000, 6000, 5));
+  iset.push_back(RAJA::RangeSegment(6000, 6001));
+  std::vector<RAJA::Index_type> all;
+  for (RAJA::Index_type i = 0; i < 1000; ++i) all.push_back(i);
+  all.insert(all.end(), list_host.begin(), list_host.end());
+  for (RAJA::Index_type i = 3000; i < 6000; i += 5) all.push_back(i);
+  all.push_back(6000);
+  return all;
+}
+
+template<typename SEGIT, typename EXEC, typename REDUCE>
+void test_host_index_set(const char* tag)
+{
+  std::vector<RAJA::Index_type> lh;
+  for (RAJA::Index_type i = 1000; i < 3000; i += 7) lh.push_back(i);
+  RAJA::TypedListSegment<RAJA::Index_type> list(lh, RAJA::resources::Host::get_default());
+  RAJA::TypedIndexSet<RAJA::RangeSegment, RAJA::TypedListSegment<RAJA::Index_type>, RAJA::RangeStrideSegment> iset;
+  const auto all = fill_index_set(iset, lh, list);
+  std::vector<int> hits(6001, 0);
+  int* hp = hits.data();
+  RAJA::ReduceSum<REDUCE, long long> s(0);
+  RAJA::forall<RAJA::ExecPolicy<SEGIT, EXEC>>(iset, [=](RAJA::Index_type i) {
+    hp[i] += 1;
+    s += i;
+  });
+  bool ok = iset.getNumSegments() == 4 && iset.getLength() == all.size();
+  long long ref = 0;
+  std::vector<int> want(6001, 0);
+  for (auto i : all)
+  {
+    want[i] = 1;
+    ref += i;
+  }
+  ok = ok && hits == want && s.get() == ref;
+  CHECK(ok, (std::string(tag) + " forall over a TypedIndexSet (range, list, stride)").c_str());
+}
+
 // ---- RAJA::launch cases (test/functional/launch in the reference) ---------------
 // One routine runs on the host (ExecPlace::HOST, seq policies, host
 // pointers) and on the device (ExecPlace::DEVICE, cdna4 policies, device
@@ -949,6 +996,43 @@ void test_cdna4_multi()
   }
 }
 
+void test_cdna4_index_set()
+{
+  std::vector<RAJA::Index_type> lh;
+  for (RAJA::Index_type i = 1000; i < 3000; i += 7) lh.push_back(i);
+  RAJA::TypedListSegment<RAJA::Index_type> list(lh, RAJA::resources::Cdna4::get_default());
+  RAJA::TypedIndexSet<RAJA::RangeSegment, RAJA::TypedListSegment<RAJA::Index_type>, RAJA::RangeStrideSegment> iset;
+  const auto all = fill_index_set(iset, lh, list);
+  int* hp = to_device(std::vector<int>(6001, 0));
+  RAJA::ReduceSum<RAJA::cdna4_reduce, long long> s(0);
+  RAJA::forall<RAJA::ExecPolicy<RAJA::seq_segit, RAJA::cdna4_exec<256>>>(iset, [=] RAJA_DEVICE(RAJA::Index_type i) {
+    hp[i] += 1;
+    s += i;
+  });
+  std::vector<int> want(6001, 0);
+  long long ref = 0;
+  for (auto i : all)
+  {
+    want[i] = 1;
+    ref += i;
+  }
+  CHECK(to_host(hp, 6001) == want && s.get() == ref, "cdna4 forall over a TypedIndexSet (range, list, stride)");
+  // segments run in order: segment 2 sees what segment 1 wrote
+  RAJA::TypedIndexSet<RAJA::RangeSegment> two;
+  two.push_back(RAJA::RangeSegment(0, 1));
+  two.push_back(RAJA::RangeSegment(1, 4097));
+  int* dp = to_device(std::vector<int>(4097, 0));
+  RAJA::forall<RAJA::ExecPolicy<RAJA::seq_segit, RAJA::cdna4_exec<256>>>(two, [=] RAJA_DEVICE(RAJA::Index_type i) {
+    dp[i] = (i == 0) ? 41 : dp[0] + 1;
+  });
+  const auto dh = to_host(dp, 4097);
+  bool ok = dh[0] == 41;
+  for (int i = 1; i < 4097; ++i) ok = ok && dh[i] == 42;
+  CHECK(ok, "cdna4 index set segments run in order");
+  dfree(hp);
+  dfree(dp);
+}
+
 #endif
 
 int main()
@@ -964,6 +1048,8 @@ int main()
   test_host_launch();
   test_host_multi<RAJA::seq_exec, RAJA::seq_multi_reduce>("seq");
   test_host_multi<RAJA::omp_parallel_for_exec, RAJA::omp_multi_reduce>("omp");
+  test_host_index_set<RAJA::seq_segit, RAJA::seq_exec, RAJA::seq_reduce>("seq_segit/seq");
+  test_host_index_set<RAJA::omp_parallel_segit, RAJA::seq_exec, RAJA::omp_reduce>("omp_segit/seq");
 #if defined(RAJA_ENABLE_CDNA4)
   test_cdna4_forall_reducers();
   test_cdna4_daxpy_and_params();
@@ -972,6 +1058,7 @@ int main()
   test_cdna4_workgroups();
   test_cdna4_launch();
   test_cdna4_multi();
+  test_cdna4_index_set();
 #endif
   std::printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
