@@ -150,7 +150,9 @@ public:
       : m_state(o.m_state), m_rows(o.m_rows), m_nbins(o.m_nbins), m_nrows(o.m_nrows),
         m_shared(o.m_shared), m_owner(false)
   {
-#if !defined(__HIP_DEVICE_COMPILE__)
+#if defined(__HIP_DEVICE_COMPILE__)
+    (void)s_fill_kernel;  // the device pass must see the fill kernel to emit it
+#else
     RAJA::cdna4::detail::LaunchInfo& li = RAJA::cdna4::detail::launch_info();
     if (li.setup && m_nbins > 0)
     {
@@ -168,11 +170,7 @@ public:
       m_rows = static_cast<T*>(p);
       m_nrows = (int)nrows;
       m_shared = nrows < (size_t)li.nblocks;
-      const size_t want = (count + 255) / 256;
-      const unsigned grid = (unsigned)(want < 4096 ? (want > 0 ? want : 1) : 4096);
-      hipLaunchKernelGGL((RAJA::cdna4::detail::multi_identity_kernel<T, Op>), dim3(grid), dim3(256), 0,
-                         li.stream, m_rows, count);
-      resources::detail::check_hip(hipGetLastError(), "multi-reduce rows fill");
+      fill_identity(m_rows, count, li.stream);
       std::lock_guard<std::mutex> g(m_state->lock);
       m_state->launches.push_back(Launch{m_rows, m_nrows, li.stream});
     }
@@ -237,6 +235,17 @@ public:
   }
 
 private:
+  static constexpr void (*s_fill_kernel)(T*, size_t) = &RAJA::cdna4::detail::multi_identity_kernel<T, Op>;
+
+  static void fill_identity(T* rows, size_t count, hipStream_t s)
+  {
+    const size_t want = (count + 255) / 256;
+    const unsigned grid = (unsigned)(want < 4096 ? (want > 0 ? want : 1) : 4096);
+    hipLaunchKernelGGL((RAJA::cdna4::detail::multi_identity_kernel<T, Op>), dim3(grid), dim3(256), 0, s,
+                       rows, count);
+    resources::detail::check_hip(hipGetLastError(), "multi-reduce rows fill");
+  }
+
   void fold_launches() const
   {
     std::vector<Launch> ls;
