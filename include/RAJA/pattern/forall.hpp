@@ -13,6 +13,7 @@
 #include "RAJA/index/IndexSet.hpp"
 #include "RAJA/pattern/params.hpp"
 #include "RAJA/policy/PolicyBase.hpp"
+#include "RAJA/util/plugins.hpp"
 #include "RAJA/util/resource.hpp"
 
 namespace RAJA
@@ -26,6 +27,7 @@ resources::EventProxy<std::decay_t<Res>> forall(Res r, Iterable&& it, Args&&... 
   static_assert(sizeof...(Args) >= 1, "forall needs a loop body");
   auto pack = detail::make_forall_param_pack(std::forward<Args>(args)...);
   auto&& body = detail::last_arg(std::forward<Args>(args)...);
+  util::PluginScope tools(util::make_context<ExecPolicy>(pack.name ? pack.name : ""));
   if (it.size() <= 0)
   {
     return resources::EventProxy<std::decay_t<Res>>(r);  // empty range: no launch
@@ -55,7 +57,10 @@ void forall_segments(Res r, seq_segit, const TypedIndexSet<Segs...>& iset, Body&
   using Each = typename detail::segit_exec<SegExec>::type;
   for (size_t k = 0; k < iset.getNumSegments(); ++k)
   {
-    iset.segmentCall(k, [&](const auto& seg) { forall<Each>(r, seg, body); });
+    iset.segmentCall(k, [&](const auto& seg) {
+      auto pack = detail::make_forall_param_pack(body);
+      if (seg.size() > 0) forall_impl(r, Each{}, seg, body, pack);
+    });
   }
   if (detail::segit_exec<SegExec>::wait_after) r.wait();
 }
@@ -71,7 +76,10 @@ void forall_segments(Res r, omp_parallel_segit, const TypedIndexSet<Segs...>& is
 #endif
   for (long k = 0; k < ns; ++k)
   {
-    iset.segmentCall((size_t)k, [&](const auto& seg) { forall<SegExec>(r, seg, body); });
+    iset.segmentCall((size_t)k, [&](const auto& seg) {
+      auto pack = detail::make_forall_param_pack(body);
+      if (seg.size() > 0) forall_impl(r, SegExec{}, seg, body, pack);
+    });
   }
 }
 }  // namespace detail
@@ -80,6 +88,7 @@ template<typename ExecPolicy, typename Res, typename... Segs, typename Body,
          std::enable_if_t<type_traits::is_resource<std::decay_t<Res>>::value>* = nullptr>
 resources::EventProxy<std::decay_t<Res>> forall(Res r, const TypedIndexSet<Segs...>& iset, Body&& body)
 {
+  util::PluginScope tools(util::make_context<ExecPolicy>(""));
   detail::forall_segments<typename ExecPolicy::seg_exec>(r, typename ExecPolicy::seg_it{}, iset,
                                                         std::forward<Body>(body));
   return resources::EventProxy<std::decay_t<Res>>(r);

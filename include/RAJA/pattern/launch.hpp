@@ -28,6 +28,7 @@
 #include "RAJA/pattern/params.hpp"
 #include "RAJA/policy/PolicyBase.hpp"
 #include "RAJA/policy/host.hpp"
+#include "RAJA/util/plugins.hpp"
 #include "RAJA/util/resource.hpp"
 
 namespace RAJA
@@ -260,6 +261,7 @@ resources::EventProxy<std::decay_t<Res>> launch(Res res, const LaunchParams& par
   using Pol = typename detail::policy_for_resource<POLICY_LIST, std::decay_t<Res>>::type;
   auto pack = detail::make_forall_param_pack(std::forward<Args>(args)...);
   auto&& body = detail::last_arg(std::forward<Args>(args)...);
+  util::PluginScope tools(util::make_context<Pol>(pack.name ? pack.name : ""));
   return LaunchExecute<Pol>::exec(res, params, body, pack);
 }
 
@@ -299,6 +301,7 @@ auto launch(const LaunchParams& params, Args&&... args)
   using Res = typename Pol::resource_type;
   auto pack = detail::make_forall_param_pack(std::forward<Args>(args)...);
   auto&& body = detail::last_arg(std::forward<Args>(args)...);
+  util::PluginScope tools(util::make_context<Pol>(pack.name ? pack.name : ""));
   return LaunchExecute<Pol>::exec(Res::get_default(), params, body, pack);
 }
 

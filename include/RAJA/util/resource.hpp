@@ -218,6 +218,9 @@ private:
 
 }  // namespace resources
 
+// RAJA::Platform, as in the reference (include/RAJA/util/types.hpp).
+using resources::Platform;
+
 namespace type_traits
 {
 template<typename T>

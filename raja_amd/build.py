@@ -98,6 +98,26 @@ def build_examples():
     return outs
 
 
+PLUGINS = os.path.join(HERE, "plugins")
+
+
+def build_plugins():
+    """Tool plugins (raja_amd/plugins/*.cpp, host code) as RAJA_PLUGINS libraries."""
+    outs = []
+    for name in sorted(os.listdir(PLUGINS)):
+        if not name.endswith(".cpp"):
+            continue
+        out = os.path.join(PLUGINS, "lib" + name[:-4] + ".so")
+        cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-I" + os.path.join(ROOT, "include"),
+               "-I/opt/rocm/include", os.path.join(PLUGINS, name), "-L/opt/rocm/lib",
+               "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("%s build failed:\n%s%s" % (name, r.stdout, r.stderr))
+        outs.append(out)
+    return outs
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
     if os.path.exists(LIB):
@@ -107,3 +127,4 @@ def clean():
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print("\n".join(build_examples()))
+    print("\n".join(build_plugins()))

@@ -3,7 +3,8 @@
 //
 // A range of n elements is cut into tiles of TILE elements from its start.
 //   tile value : 256 threads; thread t folds vectors t, t+256, ..., (64 of
-//                them, 16-byte each) into 4 interleaved accumulators, the
+//                them, 16-byte each; vector k into accumulator k % 4, in
+//                increasing k) into 4 interleaved accumulators, the
 //                accumulators pairwise, then the block tree (DPP inside each
 //                wave, the 4 wave values in order);
 //   range value: a 64-ary tree over the tile values -- each level folds
@@ -23,8 +24,18 @@ namespace rt
 using namespace RAJA::cdna4::dev;
 
 constexpr int kBlock = 256;
-constexpr int kVecPerChunk = 16;  // 16-byte loads in flight per thread
-constexpr int kChunks = 4;        // chunks per tile
+constexpr int kVecPerTile = 64;  // 16-byte vectors per thread per tile
+
+// 16-byte loads a thread keeps in flight: fewer for operators whose
+// accumulators are wide (the fused Sum/Min/Max/MinLoc, MinLoc/MaxLoc), so
+// the kernel keeps enough waves resident.  Element -> accumulator
+// assignment (vector k goes to accumulator k % 4, in increasing k) does not
+// depend on it, so neither do the result bits.
+template<typename Op>
+struct LoadsInFlight
+{
+  static constexpr int value = sizeof(typename Op::acc_t) > 8 ? 4 : 16;
+};
 
 template<typename T>
 struct VecOf
@@ -36,7 +47,7 @@ struct VecOf
 template<typename T>
 constexpr int64_t tile_elems()
 {
-  return (int64_t)kBlock * kVecPerChunk * kChunks * VecOf<T>::n;
+  return (int64_t)kBlock * kVecPerTile * VecOf<T>::n;
 }
 
 // Value of tile `tile` of the range x[0..n), in every thread of the block.
@@ -51,6 +62,9 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   using vec_t = typename VecOf<T>::type;
   constexpr int VEC = VecOf<T>::n;
   constexpr int64_t TILE = tile_elems<T>();
+  constexpr int kVecPerChunk = LoadsInFlight<Op>::value;
+  constexpr int kChunks = kVecPerTile / kVecPerChunk;
+  static_assert(kVecPerChunk % 4 == 0 && kVecPerTile % kVecPerChunk == 0, "chunking keeps k % 4");
   const Op op{};
   const int t = (int)threadIdx.x;
   const int64_t base = tile * TILE;
@@ -58,6 +72,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   const bool aligned = (((uintptr_t)(x + base)) % 16) == 0;
   if (aligned && base + TILE <= n)
   {
+#pragma unroll 1
     for (int c = 0; c < kChunks; ++c)
     {
       vec_t v[kVecPerChunk];
@@ -82,7 +97,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   }
   else
   {
-    for (int k = 0; k < kVecPerChunk * kChunks; ++k)
+    for (int k = 0; k < kVecPerTile; ++k)
     {
       const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
 #pragma unroll

@@ -37,6 +37,16 @@ def build_gpu():
     return out
 
 
+def build_plugin():
+    """tests/cpp/bin/libcounter_plugin.so: a RAJA_PLUGINS test plugin (host code)."""
+    os.makedirs(BIN, exist_ok=True)
+    out = os.path.join(BIN, "libcounter_plugin.so")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-I" + INC,
+                    os.path.join(HERE, "counter_plugin.cpp"), "-o", out], check=True)
+    return out
+
+
 if __name__ == "__main__":
     print(build_host())
     print(build_gpu())
+    print(build_plugin())

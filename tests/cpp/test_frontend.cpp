@@ -345,6 +345,47 @@ void test_host_index_set(const char* tag)
   CHECK(ok, (std::string(tag) + " forall over a TypedIndexSet (range, list, stride)").c_str());
 }
 
+// ---- tool plugins (test/integration/plugin in the reference) ------------------
+
+struct NameRecorder : public RAJA::util::PluginStrategy
+{
+  static std::vector<std::string>& names()
+  {
+    static std::vector<std::string> v;
+    return v;
+  }
+  static std::vector<RAJA::Platform>& platforms()
+  {
+    static std::vector<RAJA::Platform> v;
+    return v;
+  }
+  static long& posts()
+  {
+    static long n = 0;
+    return n;
+  }
+  void preLaunch(const RAJA::util::PluginContext& p) override
+  {
+    names().push_back(p.kernel_name);
+    platforms().push_back(p.platform);
+  }
+  void postLaunch(const RAJA::util::PluginContext&) override { ++posts(); }
+};
+static RAJA::util::PluginRegistry::add<NameRecorder> g_recorder("name-recorder", "records kernel names");
+
+template<typename EXEC>
+void test_plugins(const char* tag, RAJA::Platform want)
+{
+  const size_t before = NameRecorder::names().size();
+  const long posts_before = NameRecorder::posts();
+  RAJA::forall<EXEC>(RAJA::RangeSegment(0, 10), RAJA::Name("plugin-named-loop"), [=] RAJA_HOST_DEVICE(RAJA::Index_type) {});
+  RAJA::forall<EXEC>(RAJA::RangeSegment(0, 10), [=] RAJA_HOST_DEVICE(RAJA::Index_type) {});
+  const auto& n = NameRecorder::names();
+  const bool ok = n.size() == before + 2 && n[before] == "plugin-named-loop" && n[before + 1].empty() &&
+                  NameRecorder::platforms()[before] == want && NameRecorder::posts() == posts_before + 2;
+  CHECK(ok, (std::string(tag) + " plugins see forall with name and platform").c_str());
+}
+
 // ---- RAJA::launch cases (test/functional/launch in the reference) ---------------
 // One routine runs on the host (ExecPlace::HOST, seq policies, host
 // pointers) and on the device (ExecPlace::DEVICE, cdna4 policies, device
@@ -1046,6 +1087,7 @@ int main()
   test_host_workgroup<RAJA::seq_work>("seq");
   test_host_workgroup<RAJA::omp_work>("omp");
   test_host_launch();
+  test_plugins<RAJA::seq_exec>("seq", RAJA::Platform::host);
   test_host_multi<RAJA::seq_exec, RAJA::seq_multi_reduce>("seq");
   test_host_multi<RAJA::omp_parallel_for_exec, RAJA::omp_multi_reduce>("omp");
   test_host_index_set<RAJA::seq_segit, RAJA::seq_exec, RAJA::seq_reduce>("seq_segit/seq");
@@ -1057,9 +1099,11 @@ int main()
   test_cdna4_scans_sorts();
   test_cdna4_workgroups();
   test_cdna4_launch();
+  test_plugins<RAJA::cdna4_exec<256>>("cdna4", RAJA::Platform::cdna4);
   test_cdna4_multi();
   test_cdna4_index_set();
 #endif
+  RAJA::util::finalize_plugins();
   std::printf("%d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
 }

@@ -14,3 +14,20 @@ def test_frontend_host_policies():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failed" in r.stdout
+
+
+def test_runtime_plugin_loaded_from_raja_plugins():
+    """A plugin library named by RAJA_PLUGINS is dlopened and sees every
+    forall/launch (src/RuntimePluginLoader.cpp:94-127 in the reference)."""
+    exe = cppbuild.build_host()
+    lib = cppbuild.build_plugin()
+    for path in (lib, os.path.dirname(lib)):  # a library, or a directory of them
+        env = dict(os.environ, OMP_NUM_THREADS="4", RAJA_PLUGINS=path)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        line = [l for l in r.stdout.splitlines() if l.startswith("[counter plugin]")]
+        assert len(line) == 1, r.stdout[-2000:]
+        fields = dict(kv.split("=", 1) for kv in line[0].split()[2:])
+        assert int(fields["pre"]) == int(fields["post"]) == int(fields["capture"]) > 10
+        assert int(fields["device"]) == 0  # host build: every kernel is a host loop
+        assert "plugin-named-loop;" in fields["names"]
