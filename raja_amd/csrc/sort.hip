@@ -10,12 +10,15 @@
 //      block, wave-uniform digits added once per wave, then global atomics);
 //   2. the host reads the histograms back for large inputs and skips every
 //      pass whose digit is the same for all keys;
-//   3. each pass is one kernel: block t owns tile t, ranks its keys inside
-//      each wave by 8 ballots (peer lanes with the same digit) against
-//      per-wave LDS digit counters, publishes its per-digit counts, looks
-//      back over predecessor tiles for each digit (one thread per digit),
-//      stages keys and then values in LDS in digit order and writes them out
-//      so equal digits land in contiguous runs.  Each pass clears the
+//   3. each pass is one kernel: block t owns tile t, loads its keys, ranks
+//      them inside each wave by 8 ballots (peer lanes with the same digit)
+//      against per-wave LDS digit counters, publishes its per-digit counts,
+//      looks back over predecessor tiles for each digit (one thread per
+//      digit, 8 predecessor rows in flight per step), stages the keys in LDS
+//      in digit order and writes them out so equal digits land in
+//      contiguous runs, then loads, stages and writes the values the same
+//      way.  Loading the values only after the keys are staged keeps the
+//      kernel at 2 blocks of 8 waves per CU.  Each pass clears the
 //      descriptors of the next one (two alternating arrays), so no clearing
 //      launch or device counter is needed.
 // Ranking follows input order inside a wave (item j, then lane) and across
@@ -29,11 +32,10 @@ namespace
 {
 using namespace RAJA::cdna4::dev;
 
-constexpr int kBlock = 256;  // one thread per digit in the per-digit phases
-constexpr int kWaves = kBlock / wave_size;
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 constexpr uint32_t kCountMask = 0x3FFFFFFFu;
-// A descriptor row: 256 digit words, the tile's hint word, padding.
+// A descriptor row: 256 digit words, padded to 264 words so rows of
+// neighbouring tiles start in different cache-line sets.
 constexpr int kRowStride = kRadix + 8;
 constexpr int64_t kMaxSortN = (int64_t)kCountMask;  // counts live in 30 bits
 
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   __shared__ int32_t s_off[kRadix];
   __shared__ uint32_t s_tstart[kRadix];
   __shared__ uint32_t s_scan[2 * (kRadix / wave_size)];
-  __shared__ int64_t s_hit;
+  __shared__ uint8_t s_dig[HAS_VALS ? TILE : 1];
   K* s_keys = reinterpret_cast<K*>(s_buf);
   V* s_vals = reinterpret_cast<V*>(s_buf);
 
@@ -166,13 +168,21 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
 
   K k[ITEMS];
-  V v[HAS_VALS ? ITEMS : 1];
-#pragma unroll
-  for (int j = 0; j < ITEMS; ++j)
+  const bool full = tbase + TILE <= n;
+  if (full)
   {
-    const int64_t i = wbase + (int64_t)j * wave_size + lane;
-    k[j] = (i < n) ? keys_in[i] : K(0);
-    if (HAS_VALS) v[j] = (i < n) ? vals_in[i] : V(0);
+    const K* kp = keys_in + wbase;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) k[j] = kp[j * wave_size + lane];
+  }
+  else
+  {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+    {
+      const int64_t i = wbase + (int64_t)j * wave_size + lane;
+      k[j] = (i < n) ? keys_in[i] : K(0);
+    }
   }
   for (int i = tid; i < kRowStride; i += BLOCK) desc_next[tile * kRowStride + i] = 0u;  // next pass
   __syncthreads();
@@ -197,8 +207,9 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const uint32_t lower = (uint32_t)__popcll(peers & lt_mask);
     uint32_t cnt = 0;
     if (valid) cnt = s_whist[wave * kRadix + d];
-    rank[j] = cnt + lower;
+    rank[j] = (cnt + lower) | (d << 16);  // digit kept beside the rank
     if (valid && lower == 0) s_whist[wave * kRadix + d] = cnt + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_sched_barrier(0);  // rank one item at a time: few live masks
   }
   __syncthreads();
 
@@ -229,34 +240,12 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   }
   if (tile > 0)
   {
-    // Wave 0 finds the nearest predecessor whose hint says all of its
-    // inclusive counts are published (one load per lane, 64 tiles a step).
-    if (wave == 0)
-    {
-      int64_t top = tile - 1;
-      int64_t hit = -1;
-      while (true)
-      {
-        const int64_t t = top - lane;
-        const uint32_t h = (t < 0) ? 1u : atomic_load_agent(&desc_cur[t * kRowStride + kRadix]);
-        const uint64_t m = ballot(h != 0u);
-        if (m)
-        {
-          hit = top - __builtin_ctzll(m);
-          break;
-        }
-        top -= wave_size;
-      }
-      if (lane == 0) s_hit = hit;
-    }
-    __syncthreads();
     if (tid < kRadix)
     {
-      // Descriptors tile-1 .. hit are read with all loads in flight at once;
-      // the fold stops at the first inclusive count.  A descriptor whose kind
-      // is not yet visible is re-read (the hint carries no ordering).
+      // Descriptors tile-1, tile-2, ... are read R at a time with all loads
+      // in flight; the fold stops at the first inclusive count.  A
+      // descriptor not yet published is re-read.
       const int d = tid;
-      const int64_t hit = s_hit;
       bool done = false;
       for (int64_t t0 = tile - 1; !done; t0 -= 8)
       {
@@ -265,21 +254,19 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
         for (int u = 0; u < 8; ++u)
         {
           const int64_t t = t0 - u;
-          w[u] = (t >= 0 && t >= hit) ? atomic_load_agent(&desc_cur[t * kRowStride + d])
-                                      : (kInc << 30);
+          w[u] = (t >= 0) ? atomic_load_agent(&desc_cur[t * kRowStride + d]) : (kInc << 30);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
         {
           if (done) break;
           const int64_t t = t0 - u;
-          if (t < 0 || t < hit)
+          if (t < 0)
           {
             done = true;
             break;
           }
-          // The hinted tile itself must show its inclusive count.
-          while ((w[u] >> 30) == 0u || (t == hit && (w[u] >> 30) != kInc))
+          while ((w[u] >> 30) == 0u)
           {
             __builtin_amdgcn_s_sleep(1);
             w[u] = atomic_load_agent(&desc_cur[t * kRowStride + d]);
@@ -292,7 +279,6 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     }
   }
   __syncthreads();
-  if (tid == 0) atomic_store_agent(&row[kRadix], 1u);  // hint: inclusive counts published
   if (tid < kRadix)
   {
     const int d = tid;
@@ -314,27 +300,46 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const int64_t i = wbase + (int64_t)j * wave_size + lane;
     if (i < n)
     {
-      const uint32_t dj = digit_of<K, DESC>(k[j], shift);
-      rank[j] += s_tstart[dj] + s_whist[wave * kRadix + dj];  // tile position
+      const uint32_t dj = rank[j] >> 16;
+      rank[j] = (rank[j] & 0xFFFFu) + s_tstart[dj] + s_whist[wave * kRadix + dj];  // tile position
       s_keys[rank[j]] = k[j];
     }
   }
   __syncthreads();
+  // Values are loaded now (keys are in LDS): their latency overlaps the key scatter.
+  V v[HAS_VALS ? ITEMS : 1];
+  if (HAS_VALS)
+  {
+    if (full)
+    {
+      const V* vp = vals_in + wbase;
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) v[j] = vp[j * wave_size + lane];
+    }
+    else
+    {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j)
+      {
+        const int64_t i = wbase + (int64_t)j * wave_size + lane;
+        v[j] = (i < n) ? vals_in[i] : V(0);
+      }
+    }
+  }
   const int64_t rem = n - tbase;
   const int tile_n = (rem < TILE) ? (int)rem : TILE;
-  int32_t dst[HAS_VALS ? ITEMS : 1];
 #pragma unroll
   for (int m = 0; m < ITEMS; ++m)
   {
     const int i = tid + m * BLOCK;
-    int32_t d = -1;
     if (i < tile_n)
     {
       const K key = s_keys[i];
-      d = s_off[digit_of<K, DESC>(key, shift)] + i;
+      const uint32_t dg = digit_of<K, DESC>(key, shift);
+      const int32_t d = s_off[dg] + i;
       if (d >= 0 && (int64_t)d < n) keys_out[d] = key;  // always true for consistent histograms
+      if (HAS_VALS) s_dig[i] = (uint8_t)dg;
     }
-    if (HAS_VALS) dst[m] = d;
   }
   if (HAS_VALS)
   {
@@ -350,7 +355,11 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     for (int m = 0; m < ITEMS; ++m)
     {
       const int i = tid + m * BLOCK;
-      if (i < tile_n && dst[m] >= 0 && (int64_t)dst[m] < n) vals_out[dst[m]] = s_vals[i];
+      if (i < tile_n)
+      {
+        const int32_t d = s_off[s_dig[i]] + i;
+        if (d >= 0 && (int64_t)d < n) vals_out[d] = s_vals[i];
+      }
     }
   }
 }
