@@ -145,6 +145,12 @@ struct OpMinLoc
   {
     return (b.val < a.val || (b.val == a.val && b.loc < a.loc)) ? b : a;
   }
+  // Element i folded into a, for elements met in increasing index order
+  // (a tie keeps the earlier index, as operator() does).
+  __device__ static void accumulate(acc_t& a, T v, int64_t i)
+  {
+    if (v < a.val) a = make_valloc<T>(v, i);
+  }
   // The initial pair wins ties (seq folds the loop result in with a strict <).
   __host__ __device__ static acc_t with_init(acc_t init, acc_t r)
   {
@@ -161,6 +167,10 @@ struct OpMaxLoc
   __host__ __device__ acc_t operator()(acc_t a, acc_t b) const
   {
     return (a.val < b.val || (b.val == a.val && b.loc < a.loc)) ? b : a;
+  }
+  __device__ static void accumulate(acc_t& a, T v, int64_t i)
+  {
+    if (a.val < v) a = make_valloc<T>(v, i);
   }
   __host__ __device__ static acc_t with_init(acc_t init, acc_t r)
   {
@@ -196,6 +206,18 @@ struct OpMulti
     return r;
   }
   __host__ __device__ static acc_t with_init(acc_t, acc_t r) { return r; }
+  // Element i folded into a, for elements met in increasing index order.
+  __device__ static void accumulate(acc_t& a, double v, int64_t i)
+  {
+    a.sum += v;
+    a.mn = (v < a.mn) ? v : a.mn;
+    a.mx = (a.mx < v) ? v : a.mx;
+    if (v < a.lv)
+    {
+      a.lv = v;
+      a.ll = i;
+    }
+  }
 };
 
 }  // namespace rk

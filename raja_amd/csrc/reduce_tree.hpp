@@ -50,6 +50,21 @@ constexpr int64_t tile_elems()
   return (int64_t)kBlock * kVecPerTile * VecOf<T>::n;
 }
 
+// acc <- op(acc, lift(x_i, i)); operators whose accumulators are wide give a
+// cheaper accumulate() that is exact for elements met in increasing index
+// order, which is how each accumulator meets them.
+template<typename Op, typename T>
+__device__ __forceinline__ auto accumulate_one(typename Op::acc_t& a, T v, int64_t i, int)
+    -> decltype(Op::accumulate(a, v, i), void())
+{
+  Op::accumulate(a, v, i);
+}
+template<typename Op, typename T>
+__device__ __forceinline__ void accumulate_one(typename Op::acc_t& a, T v, int64_t i, long)
+{
+  a = Op{}(a, Op::lift(v, i));
+}
+
 // Value of tile `tile` of the range x[0..n), in every thread of the block.
 // The element -> (thread, accumulator) assignment does not depend on the
 // alignment of x; only the load width does.
@@ -90,7 +105,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
 #pragma unroll
         for (int e = 0; e < VEC; ++e)
         {
-          acc[j & 3] = op(acc[j & 3], Op::lift(v[j][e], i0 + e));
+          accumulate_one<Op>(acc[j & 3], v[j][e], i0 + e, 0);
         }
       }
     }
@@ -105,7 +120,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
       {
         if (i0 + e < n)
         {
-          acc[k & 3] = op(acc[k & 3], Op::lift(x[i0 + e], i0 + e));
+          accumulate_one<Op>(acc[k & 3], x[i0 + e], i0 + e, 0);
         }
       }
     }
