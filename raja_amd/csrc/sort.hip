@@ -418,7 +418,7 @@ int sort_core(const K* in_keys, const V* in_vals, K* keys, V* vals, int64_t n, h
     rc::TimedScope ts("radix_hist", s);
     hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(256), 0, s, hist, (int64_t)P * kRadix);
     const int64_t want = rc::ceil_div(n, (int64_t)256 * 64);
-    const int64_t cap = (int64_t)rc::num_cus() * 4;
+    const int64_t cap = (int64_t)rc::num_cus() * 16;
     const unsigned grid = (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
     hipLaunchKernelGGL((hist_kernel<K, DESC, P>), dim3(grid), dim3(256), 0, s, in_keys, n, hist,
                        desc[0], w.desc_words);
