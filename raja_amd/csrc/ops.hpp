@@ -149,7 +149,9 @@ struct OpMinLoc
   // (a tie keeps the earlier index, as operator() does).
   __device__ static void accumulate(acc_t& a, T v, int64_t i)
   {
-    if (v < a.val) a = make_valloc<T>(v, i);
+    const bool take = v < a.val;  // selects, not a branch
+    a.val = take ? v : a.val;
+    a.loc = take ? i : a.loc;
   }
   // The initial pair wins ties (seq folds the loop result in with a strict <).
   __host__ __device__ static acc_t with_init(acc_t init, acc_t r)
@@ -170,7 +172,9 @@ struct OpMaxLoc
   }
   __device__ static void accumulate(acc_t& a, T v, int64_t i)
   {
-    if (a.val < v) a = make_valloc<T>(v, i);
+    const bool take = a.val < v;
+    a.val = take ? v : a.val;
+    a.loc = take ? i : a.loc;
   }
   __host__ __device__ static acc_t with_init(acc_t init, acc_t r)
   {
@@ -212,11 +216,9 @@ struct OpMulti
     a.sum += v;
     a.mn = (v < a.mn) ? v : a.mn;
     a.mx = (a.mx < v) ? v : a.mx;
-    if (v < a.lv)
-    {
-      a.lv = v;
-      a.ll = i;
-    }
+    const bool take = v < a.lv;  // selects, not a branch
+    a.lv = take ? v : a.lv;
+    a.ll = take ? i : a.ll;
   }
 };
 

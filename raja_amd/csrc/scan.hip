@@ -5,8 +5,8 @@
 // exclusive_scan (include/RAJA/policy/hip/scan.hpp:51-274); the semantics
 // are those of include/RAJA/policy/sequential/scan.hpp:43-153.  This kernel
 // reads every element once and writes it once (8 B/elem for int32):
-//   * a tile is 4 waves x 64 lanes x V vectors of 16 bytes; wave w owns a
-//     contiguous quarter, lane l loads vector k at ((k*64 + l) * VEC), so
+//   * a tile is 8 waves x 64 lanes x V vectors of 16 bytes; wave w owns a
+//     contiguous eighth, lane l loads vector k at ((k*64 + l) * VEC), so
 //     every load and store is a fully coalesced dwordx4;
 //   * each vector is scanned in registers, vector totals are scanned across
 //     the wave with DPP (one wave scan per k), waves are combined in LDS;
@@ -27,7 +27,7 @@ namespace
 {
 using namespace RAJA::cdna4::dev;
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 512;
 constexpr int kWaves = kBlock / wave_size;
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 
