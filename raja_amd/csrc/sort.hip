@@ -137,8 +137,12 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
 // ---- 2. one onesweep pass -----------------------------------------------------
 // Tile t is block t: workgroups are dispatched in increasing id order on each
 // XCD, so every tile a block waits for is resident or done.  Keys, then
-// values, go through one LDS staging buffer so a 8192-key tile fits in
-// ~42 KiB and the output is written in digit-contiguous runs.
+// values, go through one LDS staging buffer so a 16384-key tile (512
+// threads x 32 items) fits in ~74 KiB, two blocks per CU, and the output is
+// written in digit-contiguous runs.  The per-tile cost (barriers, digit
+// scans, look-back) is paid once per 16384 keys: larger tiles measured
+// faster than more resident blocks.  The keys are staged and the values
+// loaded before the look-back, so both overlap its latency.
 template<typename K, typename V, bool DESC, bool HAS_VALS, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const K* __restrict__ keys_in, K* __restrict__ keys_out, const V* __restrict__ vals_in,
@@ -155,7 +159,6 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   __shared__ int32_t s_off[kRadix];
   __shared__ uint32_t s_tstart[kRadix];
   __shared__ uint32_t s_scan[2 * (kRadix / wave_size)];
-  __shared__ uint8_t s_dig[HAS_VALS ? TILE : 1];
   K* s_keys = reinterpret_cast<K*>(s_buf);
   V* s_vals = reinterpret_cast<V*>(s_buf);
 
@@ -166,6 +169,11 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   const int64_t tile = blockIdx.x;
   const int64_t tbase = tile * TILE;
   const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
+
+  // Item j of this lane is element wbase + j*64 + lane; the wave holds
+  // wvalid valid elements (32-bit compares below, no 64-bit indices kept).
+  const int64_t wrem = n - wbase;
+  const int wvalid = wrem <= 0 ? 0 : (wrem >= WCHUNK ? WCHUNK : (int)wrem);
 
   K k[ITEMS];
   const bool full = tbase + TILE <= n;
@@ -180,8 +188,8 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
     {
-      const int64_t i = wbase + (int64_t)j * wave_size + lane;
-      k[j] = (i < n) ? keys_in[i] : K(0);
+      const int q = j * wave_size + lane;
+      k[j] = (q < wvalid) ? keys_in[wbase + q] : K(0);
     }
   }
   for (int i = tid; i < kRowStride; i += BLOCK) desc_next[tile * kRowStride + i] = 0u;  // next pass
@@ -193,8 +201,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j)
   {
-    const int64_t i = wbase + (int64_t)j * wave_size + lane;
-    const bool valid = i < n;
+    const bool valid = j * wave_size + lane < wvalid;
     const uint32_t d = digit_of<K, DESC>(k[j], shift);
     uint64_t peers = ballot(valid);
 #pragma unroll
@@ -208,6 +215,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     uint32_t cnt = 0;
     if (valid) cnt = s_whist[wave * kRadix + d];
     rank[j] = (cnt + lower) | (d << 16);  // digit kept beside the rank
+    asm volatile("" : "+v"(rank[j]));  // one register per item from here on
     if (valid && lower == 0) s_whist[wave * kRadix + d] = cnt + (uint32_t)__popcll(peers);
     __builtin_amdgcn_sched_barrier(0);  // rank one item at a time: few live masks
   }
@@ -236,6 +244,53 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     {
       s_scan[wave] = tot_t;
       s_scan[kRadix / wave_size + wave] = tot_g;
+    }
+  }
+  __syncthreads();
+  uint32_t tstart = 0, gstart = 0;
+  if (tid < kRadix)
+  {
+    tstart = ex_t;
+    gstart = ex_g;
+    for (int w = 0; w < wave; ++w)
+    {
+      tstart += s_scan[w];
+      gstart += s_scan[kRadix / wave_size + w];
+    }
+    s_tstart[tid] = tstart;
+  }
+  __syncthreads();
+
+  // Keys: stage in LDS in digit order (tile-local offsets only), before the
+  // look-back, so the keys' registers are free while it runs.
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+  {
+    if (j * wave_size + lane < wvalid)
+    {
+      const uint32_t dj = rank[j] >> 16;
+      rank[j] = (rank[j] & 0xFFFFu) + s_tstart[dj] + s_whist[wave * kRadix + dj];  // tile position
+      s_keys[rank[j]] = k[j];
+    }
+  }
+  // Values are loaded now: their latency overlaps the look-back.
+  V v[HAS_VALS ? ITEMS : 1];
+  if (HAS_VALS)
+  {
+    if (full)
+    {
+      const V* vp = vals_in + wbase;
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) v[j] = vp[j * wave_size + lane];
+    }
+    else
+    {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j)
+      {
+        const int q = j * wave_size + lane;
+        v[j] = (q < wvalid) ? vals_in[wbase + q] : V(0);
+      }
     }
   }
   if (tile > 0)
@@ -278,56 +333,15 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       atomic_store_agent(&row[d], (kInc << 30) | (prefix + total));
     }
   }
+  if (tid < kRadix) s_off[tid] = (int32_t)(gstart + prefix) - (int32_t)tstart;
   __syncthreads();
-  if (tid < kRadix)
-  {
-    const int d = tid;
-    uint32_t tstart = ex_t, gstart = ex_g;
-    for (int w = 0; w < wave; ++w)
-    {
-      tstart += s_scan[w];
-      gstart += s_scan[kRadix / wave_size + w];
-    }
-    s_tstart[d] = tstart;
-    s_off[d] = (int32_t)(gstart + prefix) - (int32_t)tstart;
-  }
-  __syncthreads();
-
-  // Keys: stage in LDS in digit order, then write digit-contiguous runs.
-#pragma unroll
-  for (int j = 0; j < ITEMS; ++j)
-  {
-    const int64_t i = wbase + (int64_t)j * wave_size + lane;
-    if (i < n)
-    {
-      const uint32_t dj = rank[j] >> 16;
-      rank[j] = (rank[j] & 0xFFFFu) + s_tstart[dj] + s_whist[wave * kRadix + dj];  // tile position
-      s_keys[rank[j]] = k[j];
-    }
-  }
-  __syncthreads();
-  // Values are loaded now (keys are in LDS): their latency overlaps the key scatter.
-  V v[HAS_VALS ? ITEMS : 1];
-  if (HAS_VALS)
-  {
-    if (full)
-    {
-      const V* vp = vals_in + wbase;
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) v[j] = vp[j * wave_size + lane];
-    }
-    else
-    {
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j)
-      {
-        const int64_t i = wbase + (int64_t)j * wave_size + lane;
-        v[j] = (i < n) ? vals_in[i] : V(0);
-      }
-    }
-  }
   const int64_t rem = n - tbase;
   const int tile_n = (rem < TILE) ? (int)rem : TILE;
+  // Digits of the tile positions this thread writes, four per register: the
+  // value pass writes the same positions after the keys' LDS is reused.
+  uint32_t dpk[HAS_VALS ? (ITEMS + 3) / 4 : 1];
+#pragma unroll
+  for (int q = 0; q < (HAS_VALS ? (ITEMS + 3) / 4 : 1); ++q) dpk[q] = 0u;
 #pragma unroll
   for (int m = 0; m < ITEMS; ++m)
   {
@@ -338,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       const uint32_t dg = digit_of<K, DESC>(key, shift);
       const int32_t d = s_off[dg] + i;
       if (d >= 0 && (int64_t)d < n) keys_out[d] = key;  // always true for consistent histograms
-      if (HAS_VALS) s_dig[i] = (uint8_t)dg;
+      if (HAS_VALS) dpk[m >> 2] |= dg << (8 * (m & 3));
     }
   }
   if (HAS_VALS)
@@ -347,8 +361,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
     {
-      const int64_t i = wbase + (int64_t)j * wave_size + lane;
-      if (i < n) s_vals[rank[j]] = v[j];
+      if (j * wave_size + lane < wvalid) s_vals[rank[j]] = v[j];
     }
     __syncthreads();
 #pragma unroll
@@ -357,7 +370,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       const int i = tid + m * BLOCK;
       if (i < tile_n)
       {
-        const int32_t d = s_off[s_dig[i]] + i;
+        const int32_t d = s_off[(dpk[m >> 2] >> (8 * (m & 3))) & 0xFFu] + i;
         if (d >= 0 && (int64_t)d < n) vals_out[d] = s_vals[i];
       }
     }
@@ -369,7 +382,7 @@ struct SortShape
 {
   static constexpr bool wide = sizeof(K) > 4 || (HAS_VALS && sizeof(V) > 4);
   static constexpr int BLOCK = wide ? 256 : 512;
-  static constexpr int ITEMS = 16;
+  static constexpr int ITEMS = wide ? 16 : 32;
   static constexpr int64_t TILE = (int64_t)BLOCK * ITEMS;
 };
 

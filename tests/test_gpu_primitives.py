@@ -291,7 +291,10 @@ def test_sort_golden(raja, golden):
                 assert np.array_equal(to_np(k2), np.array(case["keys"])[case[comp]].astype(to_np(k2).dtype))
 
 
-SORT_SIZES = [2, 3, 255, 4095, 4096, 4097, 100003, (1 << 18) + 5, (1 << 21) + 17]
+# Around the 4096-key (64-bit) and 16384-key (32-bit) onesweep tiles and the
+# 2048-key wave chunk of the latter.
+SORT_SIZES = [2, 3, 255, 2047, 2049, 4095, 4096, 4097, 16383, 16384, 16385, 3 * 16384 + 2049,
+              100003, (1 << 18) + 5, (1 << 21) + 17]
 
 
 @pytest.mark.parametrize("bits", [32, 12, 1])
@@ -329,7 +332,7 @@ def test_sort_pairs_types_descending(raja, orc, kdt, vdt):
 
 
 def test_sort_keys_only(raja, orc):
-    for n in (17, 4097, 300001):
+    for n in (17, 4097, 16385, 300001):
         k = torch.empty(n, dtype=torch.uint32, device=DEV)
         raja.fill(raja.FILL_U32_BITS, k, seed=31, a=32)
         hk = to_np(k)
