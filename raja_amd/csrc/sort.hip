@@ -10,15 +10,14 @@
 //      block, wave-uniform digits added once per wave, then global atomics);
 //   2. the host reads the histograms back for large inputs and skips every
 //      pass whose digit is the same for all keys;
-//   3. each pass is one kernel: block t owns tile t, loads its keys, ranks
-//      them inside each wave by 8 ballots (peer lanes with the same digit)
-//      against per-wave LDS digit counters, publishes its per-digit counts,
-//      looks back over predecessor tiles for each digit (one thread per
-//      digit, 8 predecessor rows in flight per step), stages the keys in LDS
-//      in digit order and writes them out so equal digits land in
-//      contiguous runs, then loads, stages and writes the values the same
-//      way.  Loading the values only after the keys are staged keeps the
-//      kernel at 2 blocks of 8 waves per CU.  Each pass clears the
+//   3. each pass is one kernel: block t owns tile t (16384 keys of 32 bits),
+//      loads its keys, ranks them inside each wave by 8 ballots (peer lanes
+//      with the same digit) against per-wave LDS digit counters, publishes
+//      its per-digit counts, stages the keys in LDS in digit order, loads
+//      its values, looks back over predecessor tiles for each digit (one
+//      thread per digit, 8 predecessor rows in flight per step), then writes
+//      the keys out so equal digits land in contiguous runs and stages and
+//      writes the values the same way.  Each pass clears the
 //      descriptors of the next one (two alternating arrays), so no clearing
 //      launch or device counter is needed.
 // Ranking follows input order inside a wave (item j, then lane) and across
@@ -203,14 +202,23 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   {
     const bool valid = j * wave_size + lane < wvalid;
     const uint32_t d = digit_of<K, DESC>(k[j], shift);
-    uint64_t peers = ballot(valid);
+    // Peers: lanes whose digit equals this lane's, bit by bit.  sx is 0 or
+    // all ones (the bit sign-extended); a lane's bit differs from lane m's
+    // where (ballot ^ sx) has bit m set, so acc ORs those in and the peers
+    // are the valid lanes outside acc.  v_bitop3 does acc | (ballot ^ sx) in
+    // one instruction per half.
+    const uint64_t vb = ballot(valid);
+    uint32_t accl = 0u, acch = 0u;
 #pragma unroll
     for (int b = 0; b < 8; ++b)
     {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = ballot(bit);
-      peers &= bit ? bal : ~bal;
+      int32_t sx = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
+      asm volatile("" : "+v"(sx));  // keep the one-instruction extract
+      const uint64_t bal = ballot(sx != 0);
+      accl = __builtin_amdgcn_bitop3_b32(accl, (uint32_t)bal, (uint32_t)sx, 0xF6);
+      acch = __builtin_amdgcn_bitop3_b32(acch, (uint32_t)(bal >> 32), (uint32_t)sx, 0xF6);
     }
+    const uint64_t peers = vb & ~(((uint64_t)acch << 32) | accl);
     const uint32_t lower = (uint32_t)__popcll(peers & lt_mask);
     uint32_t cnt = 0;
     if (valid) cnt = s_whist[wave * kRadix + d];
