@@ -350,19 +350,47 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   uint32_t dpk[HAS_VALS ? (ITEMS + 3) / 4 : 1];
 #pragma unroll
   for (int q = 0; q < (HAS_VALS ? (ITEMS + 3) / 4 : 1); ++q) dpk[q] = 0u;
+  // Groups of G positions: the LDS reads of a group are issued together
+  // (positions past tile_n read stale LDS inside the buffer and are not
+  // written), then the offsets, then the stores.  Full tiles skip the
+  // position test.  Destinations are < n < 2^30 for consistent histograms;
+  // the unsigned test keeps an inconsistent one from writing out of range.
+  constexpr int G = 8;
+  static_assert(ITEMS % G == 0, "scatter groups");
+  const uint32_t n32 = (uint32_t)n;
+  const bool full_tile = tile_n == TILE;
+  auto scatter_keys = [&](auto full_c) {
+    constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-  for (int m = 0; m < ITEMS; ++m)
-  {
-    const int i = tid + m * BLOCK;
-    if (i < tile_n)
+    for (int m0 = 0; m0 < ITEMS; m0 += G)
     {
-      const K key = s_keys[i];
-      const uint32_t dg = digit_of<K, DESC>(key, shift);
-      const int32_t d = s_off[dg] + i;
-      if (d >= 0 && (int64_t)d < n) keys_out[d] = key;  // always true for consistent histograms
-      if (HAS_VALS) dpk[m >> 2] |= dg << (8 * (m & 3));
+      K key[G];
+      uint32_t dg[G];
+      int32_t off[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) key[g] = s_keys[tid + (m0 + g) * BLOCK];
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+      {
+        dg[g] = digit_of<K, DESC>(key[g], shift);
+        off[g] = s_off[dg[g]];
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+      {
+        const int m = m0 + g;
+        const int i = tid + m * BLOCK;
+        const uint32_t d = (uint32_t)(off[g] + i);
+        if ((FULL || i < tile_n) && d < n32) keys_out[d] = key[g];
+        if (HAS_VALS) dpk[m >> 2] |= dg[g] << (8 * (m & 3));
+      }
+      __builtin_amdgcn_sched_barrier(0);  // one group in flight: bounded registers
     }
-  }
+  };
+  if (full_tile)
+    scatter_keys(std::true_type{});
+  else
+    scatter_keys(std::false_type{});
   if (HAS_VALS)
   {
     __syncthreads();
@@ -372,16 +400,34 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       if (j * wave_size + lane < wvalid) s_vals[rank[j]] = v[j];
     }
     __syncthreads();
+    auto scatter_vals = [&](auto full_c) {
+      constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-    for (int m = 0; m < ITEMS; ++m)
-    {
-      const int i = tid + m * BLOCK;
-      if (i < tile_n)
+      for (int m0 = 0; m0 < ITEMS; m0 += G)
       {
-        const int32_t d = s_off[(dpk[m >> 2] >> (8 * (m & 3))) & 0xFFu] + i;
-        if (d >= 0 && (int64_t)d < n) vals_out[d] = s_vals[i];
+        V val[G];
+        int32_t off[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+        {
+          const int m = m0 + g;
+          val[g] = s_vals[tid + m * BLOCK];
+          off[g] = s_off[(dpk[m >> 2] >> (8 * (m & 3))) & 0xFFu];
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+        {
+          const int i = tid + (m0 + g) * BLOCK;
+          const uint32_t d = (uint32_t)(off[g] + i);
+          if ((FULL || i < tile_n) && d < n32) vals_out[d] = val[g];
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-    }
+    };
+    if (full_tile)
+      scatter_vals(std::true_type{});
+    else
+      scatter_vals(std::false_type{});
   }
 }
 
