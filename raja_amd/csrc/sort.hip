@@ -49,8 +49,10 @@ __global__ void clear_words_kernel(uint32_t* p, int64_t n)
 
 // ---- 1. all-pass digit histograms ------------------------------------------
 // Each wave streams 64 lanes x U x 16-byte vectors per iteration (U vector
-// loads in flight per lane) and counts every digit of every key in LDS; a
-// digit shared by the whole wave is added once.
+// loads in flight per lane) and counts every digit of every key with LDS
+// atomics; a digit shared by all keys of a full iteration is added once
+// (constant high digits of small keys would otherwise serialise 64 lanes on
+// one LDS word).
 template<typename K, bool DESC, int P>
 __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, int64_t n,
                                                    uint32_t* __restrict__ hist,
@@ -94,6 +96,43 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
           k[u * VEC + e] = (i < n) ? keys[i] : K(0);
         }
       }
+    }
+    if (full)
+    {
+      // A digit that is the same for all U*VEC*64 keys of the iteration is
+      // added once; the others are counted key by key.  The test: OR over
+      // the keys of (bits ^ lane 0's first bits), one ballot per digit.
+      using bits_t = typename KeyTraits<K>::bits_t;
+      const bits_t b0 = KeyTraits<K>::to_bits(k[0]);
+      bits_t f = 0;
+      if constexpr (sizeof(bits_t) == 4)
+      {
+        f = (bits_t)__builtin_amdgcn_readfirstlane((int)b0);
+      }
+      else
+      {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
+        f = ((bits_t)hi << 32) | lo;
+      }
+      bits_t x = 0;
+#pragma unroll
+      for (int q = 0; q < U * VEC; ++q) x |= KeyTraits<K>::to_bits(k[q]) ^ f;
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+      {
+        if (ballot(((x >> (8 * p)) & (bits_t)0xFF) != 0) == 0ull)
+        {
+          const uint32_t d0 = (uint32_t)((DESC ? ~f : f) >> (8 * p)) & (kRadix - 1);
+          if (lane == 0) atomicAdd(&s_hist[p * kRadix + d0], (uint32_t)PER_WAVE);
+        }
+        else
+        {
+#pragma unroll
+          for (int q = 0; q < U * VEC; ++q) atomicAdd(&s_hist[p * kRadix + digit_of<K, DESC>(k[q], 8 * p)], 1u);
+        }
+      }
+      continue;
     }
 #pragma unroll
     for (int q = 0; q < U * VEC; ++q)
