@@ -112,17 +112,31 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   }
   else
   {
-    for (int k = 0; k < kVecPerTile; ++k)
+    // Partial or unaligned tile: the same chunks with element loads guarded
+    // by i < n (all of a chunk's loads in flight), same element order; the
+    // loop ends at the first chunk that starts past n.
+#pragma unroll 1
+    for (int c = 0; c < kChunks; ++c)
     {
-      const int64_t i0 = base + ((int64_t)k * kBlock + t) * VEC;
+      T v[kVecPerChunk][VEC];
 #pragma unroll
-      for (int e = 0; e < VEC; ++e)
+      for (int j = 0; j < kVecPerChunk; ++j)
       {
-        if (i0 + e < n)
+        const int64_t i0 = base + ((int64_t)(c * kVecPerChunk + j) * kBlock + t) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v[j][e] = (i0 + e < n) ? x[i0 + e] : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < kVecPerChunk; ++j)
+      {
+        const int64_t i0 = base + ((int64_t)(c * kVecPerChunk + j) * kBlock + t) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
         {
-          accumulate_one<Op>(acc[k & 3], x[i0 + e], i0 + e, 0);
+          if (i0 + e < n) accumulate_one<Op>(acc[j & 3], v[j][e], i0 + e, 0);
         }
       }
+      if (base + (int64_t)(c + 1) * kVecPerChunk * kBlock * VEC >= n) break;
     }
   }
   acc_t a = op(op(acc[0], acc[1]), op(acc[2], acc[3]));
