@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--no-ops", action="store_true", help="skip the secondary ops")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving stream")
     ap.add_argument("--no-halo", action="store_true", help="skip the WorkGroup halo exchange")
-    ap.add_argument("--no-forall", action="store_true", help="skip daxpy + dot + MinLoc")
+    ap.add_argument("--no-forall", action="store_true", help="skip the forall example programs (daxpy + dot + MinLoc, Sum/Min/Max/MinLoc)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -149,13 +149,18 @@ def main():
         del keys0, vals0
         ops = secondary_ops(R, torch, np, dev, stream, n)
 
-    fa = None
+    fa = fm = None
     if not args.no_forall:
         fa = run_example("daxpy_dot_minloc", ["24", "20"])
         if fa:
             for k in ("daxpy", "dot_minloc_reducers", "dot_minloc_params"):
                 log("bench: forall %-20s warm %.4f ms (%.1f GB/s)  cold %.4f ms (%.1f GB/s)"
                     % (k, fa[k]["warm_ms"], fa[k]["warm_GBs"], fa[k]["cold_ms"], fa[k]["cold_GBs"]))
+        fm = run_example("forall_multi_reducers", [str(args.log2n), "10"])
+        if fm:
+            for k in ("reducers", "params"):
+                log("bench: forall Sum/Min/Max/MinLoc %-8s %.4f ms (%.1f GB/s, %.1f%% of 8 TB/s)"
+                    % (k, fm[k]["ms"], fm[k]["GBs"], 100 * fm[k]["hbm_frac"]))
 
     halo = None
     if not args.no_halo:
@@ -205,6 +210,7 @@ def main():
         "serving": serve,
         "workgroup_halo": halo,
         "forall_daxpy_dot_minloc": fa,
+        "forall_sum_min_max_minloc": fm,
     }
     print(json.dumps(line), flush=True)
 

@@ -27,14 +27,15 @@ constexpr int kBlock = 256;
 constexpr int kVecPerTile = 64;  // 16-byte vectors per thread per tile
 
 // 16-byte loads a thread keeps in flight: fewer for operators whose
-// accumulators are wide (the fused Sum/Min/Max/MinLoc, MinLoc/MaxLoc), so
-// the kernel keeps enough waves resident.  Element -> accumulator
-// assignment (vector k goes to accumulator k % 4, in increasing k) does not
-// depend on it, so neither do the result bits.
+// accumulators are wide (8 for MinLoc/MaxLoc, 4 for the fused
+// Sum/Min/Max/MinLoc), so the kernel keeps enough waves resident.  Element
+// -> accumulator assignment (vector k goes to accumulator k % 4, in
+// increasing k) does not depend on it, so neither do the result bits.
 template<typename Op>
 struct LoadsInFlight
 {
-  static constexpr int value = sizeof(typename Op::acc_t) > 8 ? 4 : 16;
+  static constexpr int value =
+      sizeof(typename Op::acc_t) > 24 ? 4 : (sizeof(typename Op::acc_t) > 8 ? 8 : 16);
 };
 
 template<typename T>
