@@ -35,7 +35,9 @@ template<typename T>
 struct ScanShape
 {
   static constexpr int VEC = 16 / sizeof(T);
-  static constexpr int V = 8;                          // vectors per lane
+  // vectors per lane: 8 for 4-byte values (16384 per tile), 12 for 8-byte
+  // values (12288 per tile: measured 0.86 against 0.97 ms for 2^28 int64)
+  static constexpr int V = sizeof(T) == 8 ? 12 : 8;
   static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
   static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
