@@ -281,6 +281,78 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
   }
 }
 
+// Segments of at most one wave chunk (S::WCHUNK elements), one wave each:
+// no descriptors, no barriers.  The element layout, per-lane folds, wave
+// scans and prefix combines are those of wave 0 of scan_kernel over a
+// one-tile segment, so every result is bit-identical to the tile path.
+template<typename T, typename Op, bool EXCLUSIVE>
+__global__ __launch_bounds__(kBlock) void short_scan_kernel(const T* __restrict__ in,
+                                                            T* __restrict__ out,
+                                                            const rs::TileRec* __restrict__ recs,
+                                                            int64_t nrec, T init)
+{
+  using S = ScanShape<T>;
+  constexpr int VEC = S::VEC;
+  constexpr int V = S::V;
+  const Op op{};
+  const int64_t ri = (int64_t)blockIdx.x * kWaves + wave_id();
+  if (ri >= nrec) return;  // whole wave; the kernel has no barrier
+  const rs::TileRec r = recs[ri];
+  const int lane = lane_id();
+  const int64_t b = r.seg_start;
+  const int64_t n = r.seg_start + r.seg_len;
+  T* const o_base = out + r.out_delta;
+
+  T v[V][VEC];
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+  {
+#pragma unroll
+    for (int e = 0; e < VEC; ++e)
+    {
+      const int64_t i = b + ((int64_t)k * wave_size + lane) * VEC + e;
+      v[k][e] = (i < n) ? in[i] : Op::ident();
+    }
+  }
+  T lane_pref[V];
+  T run = Op::ident();
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+  {
+    T t = v[k][0];
+#pragma unroll
+    for (int e = 1; e < VEC; ++e) t = op(t, v[k][e]);
+    T excl, total;
+    (void)wave_scan(t, op, Op::ident(), excl, total);
+    lane_pref[k] = op(run, excl);
+    run = op(run, total);
+  }
+  const T start = EXCLUSIVE ? init : Op::ident();
+  const T base_pref = op(start, Op::ident());
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+  {
+    T acc = op(base_pref, lane_pref[k]);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e)
+    {
+      const int64_t i = b + ((int64_t)k * wave_size + lane) * VEC + e;
+      T o;
+      if (EXCLUSIVE)
+      {
+        o = acc;
+        acc = op(acc, v[k][e]);
+      }
+      else
+      {
+        acc = op(acc, v[k][e]);
+        o = acc;
+      }
+      if (i < n) o_base[i] = o;
+    }
+  }
+}
+
 // Zero lookback descriptors (used only when the stream is being captured:
 // a replayed graph reuses its epoch, so each replay starts from clean ones).
 __global__ void clear_desc_kernel(uint64_t* p, int64_t n)
@@ -403,29 +475,46 @@ int scan_dispatch(int dtype, int op, const void* in, void* out, int64_t n, const
 }
 
 
+// Segments longer than one wave chunk go through the tile kernel, the others
+// through short_scan_kernel (one wave each), both on stream s.
+template<typename T, typename Op, bool EXCLUSIVE>
+int seg_scan_op(const T* in, T* out, raja_cdna4_segments_s* plan, T init, hipStream_t s,
+                const char* name)
+{
+  if (plan->total > 0 && (!in || !out)) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
+  const rs::TileTable* tab = nullptr;
+  RC_TRY(rs::get_table(plan, ScanShape<T>::TILE, &tab, ScanShape<T>::WCHUNK));
+  if (tab->nshort > 0)
+  {
+    rc::TimedScope ts(name, s);
+    hipLaunchKernelGGL((short_scan_kernel<T, Op, EXCLUSIVE>), dim3((unsigned)rc::ceil_div(tab->nshort, kWaves)),
+                       dim3(kBlock), 0, s, in, out, (const rs::TileRec*)tab->d_short, tab->nshort, init);
+    RC_HIP(hipGetLastError());
+  }
+  return launch_scan<T, Op, EXCLUSIVE>(in, out, plan->total, init, s, name, tab);
+}
+
 template<typename T, bool EXCLUSIVE>
 int seg_scan_typed(int op, const void* in, void* out, raja_cdna4_segments_s* plan,
                    const void* init, hipStream_t s)
 {
-  const rs::TileTable* tab = nullptr;
-  RC_TRY(rs::get_table(plan, ScanShape<T>::TILE, &tab));
   const char* nm = EXCLUSIVE ? "segmented_exclusive_scan" : "segmented_inclusive_scan";
   switch (op)
   {
     case RAJA_CDNA4_OP_PLUS:
     {
       const T iv = init ? *static_cast<const T*>(init) : rk::OpPlus<T>::ident();
-      return launch_scan<T, rk::OpPlus<T>, EXCLUSIVE>((const T*)in, (T*)out, plan->total, iv, s, nm, tab);
+      return seg_scan_op<T, rk::OpPlus<T>, EXCLUSIVE>((const T*)in, (T*)out, plan, iv, s, nm);
     }
     case RAJA_CDNA4_OP_MIN:
     {
       const T iv = init ? *static_cast<const T*>(init) : rk::OpMin<T>::ident();
-      return launch_scan<T, rk::OpMin<T>, EXCLUSIVE>((const T*)in, (T*)out, plan->total, iv, s, nm, tab);
+      return seg_scan_op<T, rk::OpMin<T>, EXCLUSIVE>((const T*)in, (T*)out, plan, iv, s, nm);
     }
     case RAJA_CDNA4_OP_MAX:
     {
       const T iv = init ? *static_cast<const T*>(init) : rk::OpMax<T>::ident();
-      return launch_scan<T, rk::OpMax<T>, EXCLUSIVE>((const T*)in, (T*)out, plan->total, iv, s, nm, tab);
+      return seg_scan_op<T, rk::OpMax<T>, EXCLUSIVE>((const T*)in, (T*)out, plan, iv, s, nm);
     }
     default:
       return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan operator %d not supported", op);

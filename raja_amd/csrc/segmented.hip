@@ -26,10 +26,11 @@
 namespace rs
 {
 
-int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out)
+int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out, int64_t short_max)
 {
   std::lock_guard<std::mutex> g(p->lock);
-  auto it = p->tables.find(tile);
+  const auto key = std::make_pair(tile, short_max);
+  auto it = p->tables.find(key);
   if (it != p->tables.end())
   {
     *out = &it->second;
@@ -37,31 +38,44 @@ int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out)
   }
   TileTable t;
   t.tile = tile;
-  std::vector<TileRec> recs;
+  std::vector<TileRec> recs, shorts;
   std::vector<uint32_t> first((size_t)p->nseg), count((size_t)p->nseg);
   for (int64_t s = 0; s < p->nseg; ++s)
   {
     const int64_t b = p->offsets[(size_t)s], len = p->offsets[(size_t)s + 1] - b;
+    if (len > 0 && len <= short_max)
+    {
+      shorts.push_back(TileRec{b, len, 0u, (uint32_t)s, 0});
+      first[(size_t)s] = (uint32_t)recs.size();
+      count[(size_t)s] = 0;
+      continue;
+    }
     const int64_t nt = rc::ceil_div(len, tile);
     first[(size_t)s] = (uint32_t)recs.size();
     count[(size_t)s] = (uint32_t)nt;
     t.max_count = std::max(t.max_count, (uint32_t)nt);
-    for (int64_t j = 0; j < nt; ++j) recs.push_back(TileRec{b, len, (uint32_t)j, (uint32_t)s});
+    for (int64_t j = 0; j < nt; ++j) recs.push_back(TileRec{b, len, (uint32_t)j, (uint32_t)s, 0});
   }
   t.ntiles = (int64_t)recs.size();
+  t.nshort = (int64_t)shorts.size();
   RC_HIP(hipMalloc(&t.d_tiles, sizeof(TileRec) * (recs.size() + 1)));
   RC_HIP(hipMalloc(&t.d_first, sizeof(uint32_t) * (first.size() + 1)));
   RC_HIP(hipMalloc(&t.d_count, sizeof(uint32_t) * (count.size() + 1)));
+  RC_HIP(hipMalloc(&t.d_short, sizeof(TileRec) * (shorts.size() + 1)));
   if (!recs.empty())
   {
     RC_HIP(hipMemcpy(t.d_tiles, recs.data(), sizeof(TileRec) * recs.size(), hipMemcpyHostToDevice));
+  }
+  if (!shorts.empty())
+  {
+    RC_HIP(hipMemcpy(t.d_short, shorts.data(), sizeof(TileRec) * shorts.size(), hipMemcpyHostToDevice));
   }
   if (!first.empty())
   {
     RC_HIP(hipMemcpy(t.d_first, first.data(), sizeof(uint32_t) * first.size(), hipMemcpyHostToDevice));
     RC_HIP(hipMemcpy(t.d_count, count.data(), sizeof(uint32_t) * count.size(), hipMemcpyHostToDevice));
   }
-  auto ins = p->tables.emplace(tile, t);
+  auto ins = p->tables.emplace(key, t);
   *out = &ins.first->second;
   return RAJA_CDNA4_SUCCESS;
 }
@@ -412,6 +426,7 @@ int raja_cdna4_segments_destroy(raja_cdna4_segments_s* p)
     if (kv.second.d_tiles) RC_HIP(hipFree(kv.second.d_tiles));
     if (kv.second.d_first) RC_HIP(hipFree(kv.second.d_first));
     if (kv.second.d_count) RC_HIP(hipFree(kv.second.d_count));
+    if (kv.second.d_short) RC_HIP(hipFree(kv.second.d_short));
   }
   for (auto& kv : p->sort_segs) RC_HIP(hipFree(kv.second));
   if (p->d_offsets) RC_HIP(hipFree(p->d_offsets));

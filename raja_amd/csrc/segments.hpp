@@ -46,7 +46,9 @@ inline void append_tiles(std::vector<TileRec>& recs, int64_t start, int64_t len,
 }
 
 // Tiles of one tile size: tile records plus, per segment, its first tile
-// and tile count.
+// and tile count.  A table built with a short-segment limit holds the tiles
+// of the longer segments only, and the short ones (1 .. limit elements) as
+// a separate list of one record each.
 struct TileTable
 {
   int64_t tile = 0;
@@ -55,6 +57,8 @@ struct TileTable
   uint32_t* d_first = nullptr;
   uint32_t* d_count = nullptr;
   uint32_t max_count = 0;
+  int64_t nshort = 0;
+  TileRec* d_short = nullptr;
 };
 
 }  // namespace rs
@@ -66,15 +70,16 @@ struct raja_cdna4_segments_s
   int64_t total = 0;
   std::vector<int64_t> offsets;  // nseg + 1, host copy
   int64_t* d_offsets = nullptr;
-  std::map<int64_t, rs::TileTable> tables;
+  std::map<std::pair<int64_t, int64_t>, rs::TileTable> tables;  // (tile, short limit)
   std::map<int64_t, rs::SortSeg*> sort_segs;  // per block shape: short-segment records
   std::mutex lock;
 };
 
 namespace rs
 {
-// Table for tile size `tile` (built on first use, synchronously).
-int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out);
+// Table for tile size `tile` (built on first use, synchronously); with
+// short_max > 0, segments of at most short_max elements go to the short list.
+int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out, int64_t short_max = 0);
 }  // namespace rs
 
 // Launcher implemented in scan.hip for segmented scans (SEG mode of the
