@@ -11,7 +11,9 @@
 //     (reduce_tree.hpp), so a segment's result is bit-identical alone and in
 //     the batch;
 //   * scan: the single-pass scan kernel over the tile table; each segment
-//     starts from init and its lookback stops at its own first tile;
+//     starts from init and its lookback stops at its own first tile.
+//     Segments of at most one wave chunk are scanned one per wave by a
+//     second kernel on the same stream (no descriptors, same bits);
 //   * sort_pairs: one block per segment of up to 8192 keys sorts it in LDS
 //     (LSD passes with the onesweep kernel's stable wave ranking); longer
 //     segments go through the onesweep sort on the same stream.
