@@ -598,8 +598,13 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
   const SortWs<K, V, HAS_VALS> w(n);
   RC_TRY(rc::ensure_tmp(c, w.tmp_bytes()));
   RC_TRY(rc::ensure_sort_desc(c, w.desc_bytes));
+  // Under stream capture the histogram readback (a host wait) cannot run:
+  // every pass is executed instead, with the same result.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RC_HIP(hipStreamIsCapturing(s, &cap));
+  uint32_t* const pinned = cap == hipStreamCaptureStatusNone ? (uint32_t*)c->pinned : nullptr;
   return sort_core<K, V, HAS_VALS, DESC>(keys, vals, keys, vals, n, s, (char*)c->tmp,
-                                         (uint32_t*)c->sort_desc, (uint32_t*)c->pinned);
+                                         (uint32_t*)c->sort_desc, pinned);
 }
 
 template<typename K>
