@@ -474,8 +474,8 @@ template<typename K, typename V, bool HAS_VALS>
 struct SortShape
 {
   static constexpr bool wide = sizeof(K) > 4 || (HAS_VALS && sizeof(V) > 4);
-  static constexpr int BLOCK = wide ? 256 : 512;
-  static constexpr int ITEMS = wide ? 16 : 32;
+  static constexpr int BLOCK = 512;
+  static constexpr int ITEMS = wide ? 16 : 32;  // 8192 / 16384 keys per tile
   static constexpr int64_t TILE = (int64_t)BLOCK * ITEMS;
 };
 

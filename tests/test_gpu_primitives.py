@@ -294,8 +294,8 @@ def test_sort_golden(raja, golden):
                 assert np.array_equal(to_np(k2), np.array(case["keys"])[case[comp]].astype(to_np(k2).dtype))
 
 
-# Around the 4096-key (64-bit) and 16384-key (32-bit) onesweep tiles and the
-# 2048-key wave chunk of the latter.
+# Around the 8192-key (64-bit keys or values) and 16384-key (32-bit) onesweep
+# tiles and the 2048-key wave chunk of the latter.
 SORT_SIZES = [2, 3, 255, 2047, 2049, 4095, 4096, 4097, 16383, 16384, 16385, 3 * 16384 + 2049,
               100003, (1 << 18) + 5, (1 << 21) + 17]
 
