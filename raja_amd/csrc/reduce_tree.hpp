@@ -96,7 +96,8 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
       for (int j = 0; j < kVecPerChunk; ++j)
       {
         const int k = c * kVecPerChunk + j;
-        v[j] = *reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC);
+        // read once: nontemporal (2^28 FP64 sum 0.363 -> 0.316 ms)
+        v[j] = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC));
       }
 #pragma unroll
       for (int j = 0; j < kVecPerChunk; ++j)

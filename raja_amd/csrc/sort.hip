@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
 #pragma unroll
       for (int u = 0; u < U; ++u)
       {
-        const vec_t x = *reinterpret_cast<const vec_t*>(keys + base + ((int64_t)u * wave_size + lane) * VEC);
+        const vec_t x = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(keys + base + ((int64_t)u * wave_size + lane) * VEC));
 #pragma unroll
         for (int e = 0; e < VEC; ++e) k[u * VEC + e] = x[e];
       }
