@@ -5,7 +5,8 @@
 // exclusive_scan (include/RAJA/policy/hip/scan.hpp:51-274); the semantics
 // are those of include/RAJA/policy/sequential/scan.hpp:43-153.  This kernel
 // reads every element once and writes it once (8 B/elem for int32):
-//   * a tile is 8 waves x 64 lanes x V vectors of 16 bytes; wave w owns a
+//   * a tile is 8 waves x 64 lanes x V vectors of 16 bytes (32768 int32 or
+//     12288 int64 for a span, 16384 int32 in tile tables); wave w owns a
 //     contiguous eighth, lane l loads vector k at ((k*64 + l) * VEC), so
 //     every load and store is a fully coalesced dwordx4;
 //   * each vector is scanned in registers, vector totals are scanned across
@@ -31,13 +32,17 @@ constexpr int kBlock = 512;
 constexpr int kWaves = kBlock / wave_size;
 constexpr uint32_t kAgg = 1u, kInc = 2u;
 
-template<typename T>
+template<typename T, bool SEG = false>
 struct ScanShape
 {
   static constexpr int VEC = 16 / sizeof(T);
-  // vectors per lane: 8 for 4-byte values (16384 per tile), 12 for 8-byte
-  // values (12288 per tile: measured 0.86 against 0.97 ms for 2^28 int64)
-  static constexpr int V = sizeof(T) == 8 ? 12 : 8;
+  // vectors per lane: for a span, 16 for 4-byte values (32768 per tile) and
+  // 12 for 8-byte values (12288 per tile); with nontemporal loads and stores
+  // these measured best (2^28 int32: 0.385 ms at 16, 0.43 at 8, 0.40 at 20;
+  // int64: 0.83 ms at 12).  Tile tables (segments, batched requests) keep 8
+  // for 4-byte values: most of their tiles are partial, and 16 was slower
+  // (20000 log-uniform segments: 0.34 against 0.28 ms).
+  static constexpr int V = sizeof(T) == 8 ? 12 : (SEG ? 8 : 16);
   static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
   static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
                                                       uint32_t epoch_tag,
                                                       const rs::TileRec* __restrict__ tiles)
 {
-  using S = ScanShape<T>;
+  using S = ScanShape<T, SEG>;
   using vec_t = typename S::vec_t;
   constexpr int VEC = S::VEC;
   constexpr int V = S::V;
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void short_scan_kernel(const T* __restrict_
                                                             const rs::TileRec* __restrict__ recs,
                                                             int64_t nrec, T init)
 {
-  using S = ScanShape<T>;
+  using S = ScanShape<T, true>;
   constexpr int VEC = S::VEC;
   constexpr int V = S::V;
   const Op op{};
@@ -484,7 +489,7 @@ int seg_scan_op(const T* in, T* out, raja_cdna4_segments_s* plan, T init, hipStr
 {
   if (plan->total > 0 && (!in || !out)) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
   const rs::TileTable* tab = nullptr;
-  RC_TRY(rs::get_table(plan, ScanShape<T>::TILE, &tab, ScanShape<T>::WCHUNK));
+  RC_TRY(rs::get_table(plan, ScanShape<T, true>::TILE, &tab, ScanShape<T, true>::WCHUNK));
   if (tab->nshort > 0)
   {
     rc::TimedScope ts(name, s);
@@ -575,7 +580,7 @@ int table_exclusive_i32(const int32_t* in_base, int32_t* out_base, const rs::Til
       in_base, out_base, 0, 0, s, "segmented_exclusive_scan", d_tiles, ntiles, ws, 1u << 2, true);
 }
 
-int64_t scan_tile_i32() { return ScanShape<int32_t>::TILE; }
+int64_t scan_tile_i32() { return ScanShape<int32_t, true>::TILE; }
 }  // namespace rws
 
 extern "C" {

@@ -170,10 +170,10 @@ def test_scan_golden(raja, golden):
         assert np.array_equal(to_np(out)[idx], np.array(case["expect"]["val"], np.int64)), case
 
 
-# Around the 16384-element (4-byte) and 12288-element (8-byte) tiles and
-# their 2048- and 1536-element wave chunks.
+# Around the 32768-element (4-byte) and 12288-element (8-byte) tiles and
+# their 4096- and 1536-element wave chunks.
 SCAN_SIZES = [1, 2, 100, 1535, 1537, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 12287, 12288, 12289,
-              16383, 16384, 16385, 36865, 65536 * 3 + 11, (1 << 22) + 1]
+              16385, 32767, 32768, 32769, 73729, 65536 * 3 + 11, (1 << 22) + 1]
 
 
 @pytest.mark.parametrize("dtype", [torch.int32, torch.int64, torch.uint32, torch.uint64])

@@ -142,8 +142,8 @@ def test_segmented_config_size(raja, orc):
 @pytest.mark.parametrize("dtype", [torch.int32, torch.int64])
 def test_segmented_scan_short_segments(raja, orc, dtype):
     """Mostly short segments (one wave chunk or less: 2048 4-byte or 1536 8-byte
-    elements) with the chunk edges, mixed with long ones: exact against the
-    oracle for plus and min."""
+    elements) with the chunk edges, mixed with long ones (tiles of 16384 4-byte
+    or 12288 8-byte elements): exact against the oracle for plus and min."""
     rng = np.random.default_rng(17)
     lens = np.floor(2.0 ** (13 * rng.random(3000))).astype(np.int64)
     lens[:12] = [0, 1, 2, 63, 64, 65, 1535, 1536, 1537, 2047, 2048, 2049]
