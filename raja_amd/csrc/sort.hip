@@ -176,8 +176,9 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
 // Tile t is block t: workgroups are dispatched in increasing id order on each
 // XCD, so every tile a block waits for is resident or done.  Keys, then
 // values, go through one LDS staging buffer so a 16384-key tile (512
-// threads x 32 items) fits in ~74 KiB, two blocks per CU, and the output is
-// written in digit-contiguous runs.  The per-tile cost (barriers, digit
+// threads x 32 items; 8192 keys, 16 items, when keys or values are 64-bit)
+// fits in ~74 KiB, two blocks per CU, and the output is written in
+// digit-contiguous runs.  The per-tile cost (barriers, digit
 // scans, look-back) is paid once per 16384 keys: larger tiles measured
 // faster than more resident blocks.  The keys are staged and the values
 // loaded before the look-back, so both overlap its latency.
