@@ -10,8 +10,8 @@
 //      block, wave-uniform digits added once per wave, then global atomics);
 //   2. the host reads the histograms back for large inputs and skips every
 //      pass whose digit is the same for all keys;
-//   3. each pass is one kernel: block t owns tile t (16384 keys of 32 bits),
-//      loads its keys, ranks them inside each wave by 8 ballots (peer lanes
+//   3. each pass is one kernel: each block owns one tile (16384 keys of 32
+//      bits; groups of 8 consecutive tiles go to one XCD), loads its keys, ranks them inside each wave by 8 ballots (peer lanes
 //      with the same digit) against per-wave LDS digit counters, publishes
 //      its per-digit counts, stages the keys in LDS in digit order, loads
 //      its values, looks back over predecessor tiles for each digit (one
@@ -173,8 +173,9 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
 }
 
 // ---- 2. one onesweep pass -----------------------------------------------------
-// Tile t is block t: workgroups are dispatched in increasing id order on each
-// XCD, so every tile a block waits for is resident or done.  Keys, then
+// Workgroups are dispatched in increasing id order on each XCD and tiles are
+// assigned in increasing order per XCD (see the tile mapping below), so every
+// tile a block waits for is resident or done.  Keys, then
 // values, go through one LDS staging buffer so a 16384-key tile (512
 // threads x 32 items; 8192 keys, 16 items, when keys or values are 64-bit)
 // fits in ~74 KiB, two blocks per CU, and the output is written in
@@ -205,7 +206,23 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   const int lane = lane_id();
   const int wave = wave_id();
   for (int i = tid; i < NW * kRadix; i += BLOCK) s_whist[i] = 0u;
-  const int64_t tile = blockIdx.x;
+  // Tile of this block.  Blocks go to the 8 XCDs round-robin (block b to
+  // XCD b % 8), and each XCD has its own L2.  Mapping 8 consecutive tiles of
+  // every group of 64 to one XCD lets that L2 merge the adjacent digit runs
+  // of neighbouring tiles (tile t's run of digit d ends where tile t+1's
+  // begins) before they reach HBM: one pass over 2^28 pairs took 0.997 ms
+  // against 1.276 ms with tile = block (0.976 ms with 32 per group of 256).
+  // The tiles of each XCD stay in increasing order, and a group is no larger
+  // than the 64 blocks one XCD holds, so the block of the lowest unfinished
+  // tile is always resident, whatever the XCD assignment; the last, partial
+  // group keeps tile = block.
+  const int64_t tile = [] {
+    constexpr int64_t KX = 8, GX = 8 * KX;
+    const int64_t b = blockIdx.x, g = b / GX;
+    if ((g + 1) * GX > (int64_t)gridDim.x) return b;
+    const int64_t r = b - g * GX;
+    return g * GX + (r & 7) * KX + (r >> 3);
+  }();
   const int64_t tbase = tile * TILE;
   const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
 
