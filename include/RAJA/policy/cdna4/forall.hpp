@@ -7,10 +7,10 @@
 // (expt parameter reductions).
 //
 // Design:
-//   * one kernel per loop body: blocks take tiles of 4*B consecutive
-//     indices, grid-stride, and a thread handles indices t, t+B, t+2B, t+3B
-//     of its tile, so each wave has four loads in flight on adjacent
-//     512-byte segments; the grid is 2048 threads per CU (or fixed by
+//   * one kernel per loop body: blocks take tiles of 6*B consecutive
+//     indices, grid-stride, and a thread handles indices t, t+B, ..., t+5B
+//     of its tile, so each wave has six loads in flight on adjacent
+//     512-byte segments; the grid is 1024 threads per CU (or fixed by
 //     cdna4_exec_grid), never larger than the range needs, so a reducer's
 //     result depends on the range, the block size and the CU count only
 //     (not on how many registers the body takes);
@@ -348,13 +348,16 @@ void resolve_params(std::tuple<Ps...>& ps, std::index_sequence<I...>)
 
 // ---- the forall kernel -------------------------------------------------------
 
-constexpr int kForallItems = 4;  // indices per thread per tile
+constexpr int kForallItems = 6;  // indices per thread per tile
 
-// Blocks of a forall grid: 2048 threads per CU whatever the kernel's
+// Blocks of a forall grid: 1024 threads per CU whatever the kernel's
 // occupancy, so the grid (and the reducers' fold) does not depend on the body.
+// Measured with B = 256 (indices per thread / threads per CU): one forall
+// with four reducers over 2^28 FP64 0.399 ms at 4/2048, 0.367 at 6/1024,
+// 0.347 at 8/512; daxpy over 2^24 0.074, 0.071 and 0.103 ms.
 inline Index_type forall_grid_cap(int block)
 {
-  return (Index_type)raja_cdna4_num_cus() * (Index_type)std::max(1, 2048 / block);
+  return (Index_type)raja_cdna4_num_cus() * (Index_type)std::max(1, 1024 / block);
 }
 
 template<int BLOCK, typename Seg, typename Body, typename Reducers>
@@ -363,8 +366,6 @@ __global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, Reduc
 {
   Body b(body);  // device-private copy: its reducers fold per block at scope end
   auto vops = RAJA::detail::make_valops(reducers);
-  // 4 indices per thread per tile (measured: one forall with four reducers
-  // over 2^28 FP64 0.43 -> 0.39 ms, daxpy 2^24 0.082 -> 0.074 ms).
   constexpr int U = kForallItems;
   const Index_type tstride = (Index_type)gridDim.x * BLOCK * U;
   for (Index_type base = (Index_type)blockIdx.x * BLOCK * U; base < len; base += tstride)
