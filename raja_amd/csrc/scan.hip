@@ -19,6 +19,8 @@
 //   * descriptors carry a per-stream epoch chosen by the host, so they never
 //     need clearing between calls; under graph capture (where the epoch is
 //     frozen into the graph) a clear kernel runs first.
+#include <type_traits>
+
 #include "ops.hpp"
 #include "runtime.hpp"
 #include "segments.hpp"
@@ -46,7 +48,52 @@ struct ScanShape
   static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
   static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+  // The same 16 bytes at the element's own alignment: a tile of a segment
+  // starts at any element.  16-byte global accesses need only 4-byte
+  // alignment on CDNA.
+  typedef T uvec_t __attribute__((ext_vector_type(16 / sizeof(T)), aligned(sizeof(T))));
 };
+
+// Elements [i0, i0 + VEC) of a lane's vector where only indices < n exist:
+// one 16-byte access when the whole vector is in range, element accesses at
+// the range's end.
+template<typename S, typename T>
+__device__ __forceinline__ void load_vec_guarded(const T* in, int64_t i0, int64_t n,
+                                                 T (&v)[S::VEC], T ident)
+{
+  if (i0 + S::VEC <= n)
+  {
+    const typename S::uvec_t x = *reinterpret_cast<const typename S::uvec_t*>(in + i0);
+#pragma unroll
+    for (int e = 0; e < S::VEC; ++e) v[e] = x[e];
+  }
+  else
+  {
+#pragma unroll
+    for (int e = 0; e < S::VEC; ++e) v[e] = (i0 + e < n) ? in[i0 + e] : ident;
+  }
+}
+
+template<typename S, typename T>
+__device__ __forceinline__ void store_vec_guarded(T* out, int64_t i0, int64_t n,
+                                                  const T (&o)[S::VEC])
+{
+  if (i0 + S::VEC <= n)
+  {
+    typename S::uvec_t x;
+#pragma unroll
+    for (int e = 0; e < S::VEC; ++e) x[e] = o[e];
+    *reinterpret_cast<typename S::uvec_t*>(out + i0) = x;
+  }
+  else
+  {
+#pragma unroll
+    for (int e = 0; e < S::VEC; ++e)
+    {
+      if (i0 + e < n) out[i0 + e] = o[e];
+    }
+  }
+}
 
 // Descriptor of one tile: one 64-bit word per 32-bit half of the value, each
 // holding (status << 32 | half).  status = epoch << 2 | kind.
@@ -165,9 +212,11 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
   const int lane = lane_id();
   const int wave = wave_id();
   const int64_t wbase = tbase + (int64_t)wave * S::WCHUNK;
-  const bool aligned = SEG ? ((((uintptr_t)(in + tbase)) | ((uintptr_t)(out + tbase))) % 16) == 0
-                           : ALIGNED;
-  const bool full = aligned && tbase + S::TILE <= nend;
+  // A segment's tiles start at any element: they take 16-byte accesses at the
+  // element's alignment.  A span's tiles take 16-byte aligned ones when the
+  // host found both pointers aligned.
+  using lvec_t = std::conditional_t<SEG, typename S::uvec_t, vec_t>;
+  const bool full = (SEG || ALIGNED) && tbase + S::TILE <= nend;
   n = nend;
 
   T v[V][VEC];
@@ -177,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
     for (int k = 0; k < V; ++k)
     {
       // streamed once in and once out: nontemporal loads and stores (about 3.5 % faster)
-      const vec_t x = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(in + wbase + ((int64_t)k * wave_size + lane) * VEC));
+      const lvec_t x = __builtin_nontemporal_load(reinterpret_cast<const lvec_t*>(in + wbase + ((int64_t)k * wave_size + lane) * VEC));
 #pragma unroll
       for (int e = 0; e < VEC; ++e) v[k][e] = x[e];
     }
@@ -187,12 +236,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
 #pragma unroll
     for (int k = 0; k < V; ++k)
     {
-#pragma unroll
-      for (int e = 0; e < VEC; ++e)
-      {
-        const int64_t i = wbase + ((int64_t)k * wave_size + lane) * VEC + e;
-        v[k][e] = (i < n) ? in[i] : Op::ident();
-      }
+      load_vec_guarded<S>(in, wbase + ((int64_t)k * wave_size + lane) * VEC, n, v[k], Op::ident());
     }
   }
 
@@ -266,10 +310,10 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
 #pragma unroll
     for (int k = 0; k < V; ++k)
     {
-      vec_t x;
+      lvec_t x;
 #pragma unroll
       for (int e = 0; e < VEC; ++e) x[e] = o[k][e];
-      __builtin_nontemporal_store(x, reinterpret_cast<vec_t*>(out + wbase + ((int64_t)k * wave_size + lane) * VEC));
+      __builtin_nontemporal_store(x, reinterpret_cast<lvec_t*>(out + wbase + ((int64_t)k * wave_size + lane) * VEC));
     }
   }
   else
@@ -277,12 +321,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
 #pragma unroll
     for (int k = 0; k < V; ++k)
     {
-#pragma unroll
-      for (int e = 0; e < VEC; ++e)
-      {
-        const int64_t i = wbase + ((int64_t)k * wave_size + lane) * VEC + e;
-        if (i < n) out[i] = o[k][e];
-      }
+      store_vec_guarded<S>(out, wbase + ((int64_t)k * wave_size + lane) * VEC, n, o[k]);
     }
   }
 }
@@ -313,12 +352,7 @@ __global__ __launch_bounds__(kBlock) void short_scan_kernel(const T* __restrict_
 #pragma unroll
   for (int k = 0; k < V; ++k)
   {
-#pragma unroll
-    for (int e = 0; e < VEC; ++e)
-    {
-      const int64_t i = b + ((int64_t)k * wave_size + lane) * VEC + e;
-      v[k][e] = (i < n) ? in[i] : Op::ident();
-    }
+    load_vec_guarded<S>(in, b + ((int64_t)k * wave_size + lane) * VEC, n, v[k], Op::ident());
   }
   T lane_pref[V];
   T run = Op::ident();
@@ -339,23 +373,22 @@ __global__ __launch_bounds__(kBlock) void short_scan_kernel(const T* __restrict_
   for (int k = 0; k < V; ++k)
   {
     T acc = op(base_pref, lane_pref[k]);
+    T o[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e)
     {
-      const int64_t i = b + ((int64_t)k * wave_size + lane) * VEC + e;
-      T o;
       if (EXCLUSIVE)
       {
-        o = acc;
+        o[e] = acc;
         acc = op(acc, v[k][e]);
       }
       else
       {
         acc = op(acc, v[k][e]);
-        o = acc;
+        o[e] = acc;
       }
-      if (i < n) o_base[i] = o;
     }
+    store_vec_guarded<S>(o_base, b + ((int64_t)k * wave_size + lane) * VEC, n, o);
   }
 }
 
