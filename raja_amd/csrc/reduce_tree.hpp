@@ -43,6 +43,9 @@ struct VecOf
 {
   static constexpr int n = 16 / sizeof(T);
   typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+  // The same 16 bytes at the element's own alignment (a segment starts at
+  // any element); 16-byte global accesses need only 4-byte alignment on CDNA.
+  typedef T utype __attribute__((ext_vector_type(16 / sizeof(T)), aligned(sizeof(T))));
 };
 
 template<typename T>
@@ -68,7 +71,7 @@ __device__ __forceinline__ void accumulate_one(typename Op::acc_t& a, T v, int64
 
 // Value of tile `tile` of the range x[0..n), in every thread of the block.
 // The element -> (thread, accumulator) assignment does not depend on the
-// alignment of x; only the load width does.
+// alignment of x: loads are 16 bytes at the element's alignment.
 template<typename T, typename Op>
 __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x, int64_t n,
                                                         int64_t tile,
@@ -76,6 +79,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
 {
   using acc_t = typename Op::acc_t;
   using vec_t = typename VecOf<T>::type;
+  using uvec_t = typename VecOf<T>::utype;
   constexpr int VEC = VecOf<T>::n;
   constexpr int64_t TILE = tile_elems<T>();
   constexpr int kVecPerChunk = LoadsInFlight<Op>::value;
@@ -85,8 +89,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   const int t = (int)threadIdx.x;
   const int64_t base = tile * TILE;
   acc_t acc[4] = {Op::ident(), Op::ident(), Op::ident(), Op::ident()};
-  const bool aligned = (((uintptr_t)(x + base)) % 16) == 0;
-  if (aligned && base + TILE <= n)
+  if (base + TILE <= n)
   {
 #pragma unroll 1
     for (int c = 0; c < kChunks; ++c)
@@ -97,7 +100,7 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
       {
         const int k = c * kVecPerChunk + j;
         // read once: nontemporal (2^28 FP64 sum 0.363 -> 0.316 ms)
-        v[j] = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC));
+        v[j] = __builtin_nontemporal_load(reinterpret_cast<const uvec_t*>(x + base + ((int64_t)k * kBlock + t) * VEC));
       }
 #pragma unroll
       for (int j = 0; j < kVecPerChunk; ++j)
@@ -114,9 +117,10 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
   }
   else
   {
-    // Partial or unaligned tile: the same chunks with element loads guarded
-    // by i < n (all of a chunk's loads in flight), same element order; the
-    // loop ends at the first chunk that starts past n.
+    // Partial tile: the same chunks, a vector load where the whole vector is
+    // in range and element loads guarded by i < n at the end (all of a
+    // chunk's loads in flight), same element order; the loop ends at the
+    // first chunk that starts past n.
 #pragma unroll 1
     for (int c = 0; c < kChunks; ++c)
     {
@@ -125,8 +129,17 @@ __device__ __forceinline__ typename Op::acc_t fold_tile(const T* __restrict__ x,
       for (int j = 0; j < kVecPerChunk; ++j)
       {
         const int64_t i0 = base + ((int64_t)(c * kVecPerChunk + j) * kBlock + t) * VEC;
+        if (i0 + VEC <= n)
+        {
+          const uvec_t q = *reinterpret_cast<const uvec_t*>(x + i0);
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) v[j][e] = (i0 + e < n) ? x[i0 + e] : T(0);
+          for (int e = 0; e < VEC; ++e) v[j][e] = q[e];
+        }
+        else
+        {
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) v[j][e] = (i0 + e < n) ? x[i0 + e] : T(0);
+        }
       }
 #pragma unroll
       for (int j = 0; j < kVecPerChunk; ++j)
