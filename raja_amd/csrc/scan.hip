@@ -6,7 +6,7 @@
 // are those of include/RAJA/policy/sequential/scan.hpp:43-153.  This kernel
 // reads every element once and writes it once (8 B/elem for int32):
 //   * a tile is 8 waves x 64 lanes x V vectors of 16 bytes (32768 int32 or
-//     12288 int64 for a span, 16384 int32 in tile tables); wave w owns a
+//     12288 int64 for a span, 16384 or 8192 int32 in tile tables); wave w owns a
 //     contiguous eighth, lane l loads vector k at ((k*64 + l) * VEC), so
 //     every load and store is a fully coalesced dwordx4;
 //   * each vector is scanned in registers, vector totals are scanned across
@@ -33,18 +33,24 @@ using namespace RAJA::cdna4::dev;
 constexpr int kBlock = 512;
 constexpr int kWaves = kBlock / wave_size;
 constexpr uint32_t kAgg = 1u, kInc = 2u;
+// Segment plans of 4-byte values whose mean segment length is below this
+// take the short-segment tile shape.
+constexpr int64_t kShortMeanLen = 65536;
 
-template<typename T, bool SEG = false>
+// Tile shapes: MODE 0 a span, MODE 1 a tile table (batched requests, long
+// segments), MODE 2 a tile table of mostly short segments.
+template<typename T, int MODE = 0>
 struct ScanShape
 {
   static constexpr int VEC = 16 / sizeof(T);
   // vectors per lane: for a span, 16 for 4-byte values (32768 per tile) and
   // 12 for 8-byte values (12288 per tile); with nontemporal loads and stores
   // these measured best (2^28 int32: 0.385 ms at 16, 0.43 at 8, 0.40 at 20;
-  // int64: 0.83 ms at 12).  Tile tables (segments, batched requests) keep 8
-  // for 4-byte values: most of their tiles are partial, and 16 was slower
-  // (20000 log-uniform segments: 0.34 against 0.28 ms).
-  static constexpr int V = sizeof(T) == 8 ? 12 : (SEG ? 8 : 16);
+  // int64: 0.83 ms at 12).  Tile tables keep 8 for 4-byte values, and 4 when
+  // their segments are short on average: most of their tiles are partial
+  // (20000 log-uniform segments of mean length 5800, int32: 0.160 ms at 4,
+  // 0.170 at 6, 0.185 at 8, 0.200 at 12, 0.22 at 16).
+  static constexpr int V = sizeof(T) == 8 ? 12 : (MODE == 0 ? 16 : (MODE == 1 ? 8 : 4));
   static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
   static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
@@ -181,15 +187,17 @@ __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, int64_t
 
 // Tile t is block t.  Workgroups are dispatched in increasing id order on
 // each XCD, so every tile a block waits for is already resident or done.
-// SEG: the tiles of many ranges (segments) come from a tile table; each
-// range starts from `init` and its lookback stops at its own first tile.
-template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED, bool SEG>
+// MODE != 0 (SEG): the tiles of many ranges (segments) come from a tile
+// table; each range starts from `init` and its lookback stops at its own
+// first tile.
+template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED, int MODE>
 __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64_t n, T init,
                                                       Desc<T>* __restrict__ desc,
                                                       uint32_t epoch_tag,
                                                       const rs::TileRec* __restrict__ tiles)
 {
-  using S = ScanShape<T, SEG>;
+  constexpr bool SEG = MODE != 0;
+  using S = ScanShape<T, MODE>;
   using vec_t = typename S::vec_t;
   constexpr int VEC = S::VEC;
   constexpr int V = S::V;
@@ -330,13 +338,13 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64
 // no descriptors, no barriers.  The element layout, per-lane folds, wave
 // scans and prefix combines are those of wave 0 of scan_kernel over a
 // one-tile segment, so every result is bit-identical to the tile path.
-template<typename T, typename Op, bool EXCLUSIVE>
+template<typename T, typename Op, bool EXCLUSIVE, int MODE>
 __global__ __launch_bounds__(kBlock) void short_scan_kernel(const T* __restrict__ in,
                                                             T* __restrict__ out,
                                                             const rs::TileRec* __restrict__ recs,
                                                             int64_t nrec, T init)
 {
-  using S = ScanShape<T, true>;
+  using S = ScanShape<T, MODE>;
   constexpr int VEC = S::VEC;
   constexpr int V = S::V;
   const Op op{};
@@ -403,14 +411,22 @@ __global__ void clear_desc_kernel(uint64_t* p, int64_t n)
   }
 }
 
+// The table shape of a segment plan: short tiles (MODE 2) when its
+// segments are short on average; 8-byte values have one table shape.
+template<typename T>
+int table_mode(const raja_cdna4_segments_s* plan)
+{
+  return (sizeof(T) == 4 && plan->nseg > 0 && plan->total < plan->nseg * kShortMeanLen) ? 2 : 1;
+}
+
 // One scan launch.  tiles == nullptr: one span of n elements (ntiles
-// tiles); otherwise SEG mode over ntiles tile records.  desc holds ntiles
-// descriptors; with `clear` they are zeroed first (required whenever the
-// memory may hold this epoch tag from an earlier use).
+// tiles); otherwise tile table records of shape `mode` (1 or 2).  desc holds
+// ntiles descriptors; with `clear` they are zeroed first (required whenever
+// the memory may hold this epoch tag from an earlier use).
 template<typename T, typename Op, bool EXCLUSIVE>
 int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
                      const rs::TileRec* tiles, int64_t ntiles, void* desc, uint32_t epoch_tag,
-                     bool clear)
+                     bool clear, int mode = 1)
 {
   const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
   if (clear)
@@ -420,20 +436,32 @@ int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, cons
                        dim3(256), 0, s, (uint64_t*)desc, words);
   }
   rc::TimedScope ts(name, s);
-  if (tiles)
+  if (tiles && mode == 2)
   {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, true>), dim3((unsigned)ntiles),
+    if constexpr (sizeof(T) == 4)
+    {
+      hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, 2>), dim3((unsigned)ntiles),
+                         dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag, tiles);
+    }
+    else
+    {
+      return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "short-segment tiles are for 4-byte values");
+    }
+  }
+  else if (tiles)
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, 1>), dim3((unsigned)ntiles),
                        dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag, tiles);
   }
   else if (aligned)
   {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, false>), dim3((unsigned)ntiles),
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, 0>), dim3((unsigned)ntiles),
                        dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag,
                        (const rs::TileRec*)nullptr);
   }
   else
   {
-    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, false>), dim3((unsigned)ntiles),
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, 0>), dim3((unsigned)ntiles),
                        dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag,
                        (const rs::TileRec*)nullptr);
   }
@@ -446,7 +474,7 @@ int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, cons
 // stream's scratch.
 template<typename T, typename Op, bool EXCLUSIVE>
 int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
-                const rs::TileTable* table = nullptr)
+                const rs::TileTable* table = nullptr, int mode = 1)
 {
   if (n < 0) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "negative length");
   if (n == 0) return RAJA_CDNA4_SUCCESS;  // empty range: no-op (pattern/scan.hpp:72-75)
@@ -468,7 +496,7 @@ int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const cha
   RC_HIP(hipStreamIsCapturing(s, &cap));
   return launch_scan_core<T, Op, EXCLUSIVE>(
       in, out, n, init, s, name, table ? (const rs::TileRec*)table->d_tiles : nullptr, ntiles,
-      c->scan_desc, epoch_tag, cap != hipStreamCaptureStatusNone);
+      c->scan_desc, epoch_tag, cap != hipStreamCaptureStatusNone, mode);
 }
 
 template<typename T, bool EXCLUSIVE>
@@ -516,21 +544,34 @@ int scan_dispatch(int dtype, int op, const void* in, void* out, int64_t n, const
 
 // Segments longer than one wave chunk go through the tile kernel, the others
 // through short_scan_kernel (one wave each), both on stream s.
+template<typename T, typename Op, bool EXCLUSIVE, int MODE>
+int seg_scan_shape(const T* in, T* out, raja_cdna4_segments_s* plan, T init, hipStream_t s,
+                   const char* name)
+{
+  using S = ScanShape<T, MODE>;
+  const rs::TileTable* tab = nullptr;
+  RC_TRY(rs::get_table(plan, S::TILE, &tab, S::WCHUNK));
+  if (tab->nshort > 0)
+  {
+    rc::TimedScope ts(name, s);
+    hipLaunchKernelGGL((short_scan_kernel<T, Op, EXCLUSIVE, MODE>),
+                       dim3((unsigned)rc::ceil_div(tab->nshort, kWaves)), dim3(kBlock), 0, s, in,
+                       out, (const rs::TileRec*)tab->d_short, tab->nshort, init);
+    RC_HIP(hipGetLastError());
+  }
+  return launch_scan<T, Op, EXCLUSIVE>(in, out, plan->total, init, s, name, tab, MODE);
+}
+
 template<typename T, typename Op, bool EXCLUSIVE>
 int seg_scan_op(const T* in, T* out, raja_cdna4_segments_s* plan, T init, hipStream_t s,
                 const char* name)
 {
   if (plan->total > 0 && (!in || !out)) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
-  const rs::TileTable* tab = nullptr;
-  RC_TRY(rs::get_table(plan, ScanShape<T, true>::TILE, &tab, ScanShape<T, true>::WCHUNK));
-  if (tab->nshort > 0)
+  if constexpr (sizeof(T) == 4)
   {
-    rc::TimedScope ts(name, s);
-    hipLaunchKernelGGL((short_scan_kernel<T, Op, EXCLUSIVE>), dim3((unsigned)rc::ceil_div(tab->nshort, kWaves)),
-                       dim3(kBlock), 0, s, in, out, (const rs::TileRec*)tab->d_short, tab->nshort, init);
-    RC_HIP(hipGetLastError());
+    if (table_mode<T>(plan) == 2) return seg_scan_shape<T, Op, EXCLUSIVE, 2>(in, out, plan, init, s, name);
   }
-  return launch_scan<T, Op, EXCLUSIVE>(in, out, plan->total, init, s, name, tab);
+  return seg_scan_shape<T, Op, EXCLUSIVE, 1>(in, out, plan, init, s, name);
 }
 
 template<typename T, bool EXCLUSIVE>
@@ -613,7 +654,7 @@ int table_exclusive_i32(const int32_t* in_base, int32_t* out_base, const rs::Til
       in_base, out_base, 0, 0, s, "segmented_exclusive_scan", d_tiles, ntiles, ws, 1u << 2, true);
 }
 
-int64_t scan_tile_i32() { return ScanShape<int32_t, true>::TILE; }
+int64_t scan_tile_i32() { return ScanShape<int32_t, 1>::TILE; }
 }  // namespace rws
 
 extern "C" {

@@ -139,16 +139,24 @@ def test_segmented_config_size(raja, orc):
     plan.close()
 
 
+@pytest.mark.parametrize("mix", ["short_mean", "long_mean"])
 @pytest.mark.parametrize("dtype", [torch.int32, torch.int64])
-def test_segmented_scan_short_segments(raja, orc, dtype):
-    """Mostly short segments (one wave chunk or less: 2048 4-byte or 1536 8-byte
-    elements) with the chunk edges, mixed with long ones (tiles of 16384 4-byte
-    or 12288 8-byte elements): exact against the oracle for plus and min."""
+def test_segmented_scan_short_segments(raja, orc, dtype, mix):
+    """Short segments (one wave chunk or less: 1024 or 2048 4-byte, 1536 8-byte
+    elements) with the chunk and tile edges, mixed with long ones.  4-byte
+    plans whose mean length is below 65536 take tiles of 8192 elements
+    (chunks of 1024), the others tiles of 16384 (chunks of 2048); 8-byte
+    plans take tiles of 12288.  Exact against the oracle for plus and min."""
     rng = np.random.default_rng(17)
-    lens = np.floor(2.0 ** (13 * rng.random(3000))).astype(np.int64)
-    lens[:12] = [0, 1, 2, 63, 64, 65, 1535, 1536, 1537, 2047, 2048, 2049]
-    lens[12:16] = [20000, 16385, 12289, 70000]
+    edges = [0, 1, 2, 63, 64, 65, 1023, 1024, 1025, 1535, 1536, 1537, 2047, 2048, 2049,
+             8191, 8192, 8193, 12287, 12288, 12289, 16383, 16384, 16385, 20000, 70000]
+    if mix == "short_mean":
+        lens = np.floor(2.0 ** (13 * rng.random(3000))).astype(np.int64)
+        lens[:len(edges)] = edges
+    else:
+        lens = np.array(edges + [300000] * 12, dtype=np.int64)
     rng.shuffle(lens)
+    assert (lens.mean() < 65536) == (mix == "short_mean")
     off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     n = int(off[-1])
     x32 = torch.empty(n, dtype=torch.int32, device=DEV)
