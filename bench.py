@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--no-ops", action="store_true", help="skip the secondary ops")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving stream")
     ap.add_argument("--no-halo", action="store_true", help="skip the WorkGroup halo exchange")
+    ap.add_argument("--no-per-segment", action="store_true",
+                    help="skip the one-call-per-segment comparison (tens of thousands of launches; for counter runs)")
     ap.add_argument("--no-forall", action="store_true", help="skip the forall example programs (daxpy + dot + MinLoc, Sum/Min/Max/MinLoc)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
@@ -147,7 +149,7 @@ def main():
     ops = {}
     if not args.no_ops:
         del keys0, vals0
-        ops = secondary_ops(R, torch, np, dev, stream, n)
+        ops = secondary_ops(R, torch, np, dev, stream, n, not args.no_per_segment)
 
     fa = fm = None
     if not args.no_forall:
@@ -231,7 +233,7 @@ def committed_traffic(log2n):
     return (int(m.group(1)), TRAFFIC_FILE) if m else (None, None)
 
 
-def secondary_ops(R, torch, np, dev, stream, n):
+def secondary_ops(R, torch, np, dev, stream, n, per_segment=True):
     out = {}
 
     def rec(name, times, nbytes, extra=None):
@@ -280,11 +282,11 @@ def secondary_ops(R, torch, np, dev, stream, n):
         {"passes_executed": 2, "bytes_per_elem": 36, "pass_count_bound_full_P4_GBs_equiv":
          round(n * 68 / (statistics.median(t) * 1e-3) / 1e9, 1)})
     del k0, v0, k, v
-    out.update(segmented_ops(R, torch, np, dev, stream, rec))
+    out.update(segmented_ops(R, torch, np, dev, stream, rec, per_segment))
     return out
 
 
-def segmented_ops(R, torch, np, dev, stream, rec):
+def segmented_ops(R, torch, np, dev, stream, rec, per_segment=True):
     """20000 segments of length clamp(floor(2^(16u)), 1, 65536), u uniform
     (seed 6): batched FP64 sum and int32 exclusive scan over one plan, and
     the same work as one C-ABI call per segment issued from a loop."""
@@ -309,6 +311,9 @@ def segmented_ops(R, torch, np, dev, stream, rec):
     tb = event_time(lambda: R.segmented_exclusive_scan(plan, xi, oi, R.operators.plus, 0), stream, reps=9)
     rec("segmented_exclusive_scan_i32_20000", tb, 8 * total, {"elements": total})
 
+    if not per_segment:
+        plan.close()
+        return out
     L = R.lib()
     sp = ctypes.c_void_p(stream.cuda_stream)
     zd, zi = ctypes.c_double(0.0), ctypes.c_int32(0)
