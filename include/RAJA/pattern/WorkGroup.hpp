@@ -261,69 +261,118 @@ struct dispatch_list<direct_dispatch<RC...>>
   using type = camp::list<RC...>;
 };
 
-// Runner: specialised per exec policy.  Provides
-//   static void run(Storage&, RunData&, Resource, order tag, Args...)
-template<typename ExecPolicy, typename OrderPolicy, typename DispatchPolicy, typename Index,
-         typename... Args>
-struct WorkRunner;
-
-// Whether the exec policy copies loop bodies to the GPU (bytewise).
-template<typename ExecPolicy>
-struct copies_bodies_to_device : std::false_type
+// Records one loop in the storage (the work every runner's enqueue does):
+// empty segments are ignored; with direct_dispatch the body's position in
+// the dispatch list selects its call at run time.
+template<bool ON_DEVICE, typename Dispatch, typename Storage, typename Segment, typename Body>
+void enqueue_loop(Storage& storage, Segment&& seg, Body&& body)
 {
-};
-
-// Per-group data a runner keeps between runs (device tables, etc.).
-template<typename ExecPolicy>
-struct RunData
-{
-  void release() {}
-};
-
-// ---- host runners: seq_work and omp_work ----------------------------------
-
-template<typename Order, typename Dispatch, typename Index, typename... Args>
-struct WorkRunner<seq_work, Order, Dispatch, Index, Args...>
-{
-  using resource_type = resources::Host;
-  template<typename Storage>
-  static void run(Storage& s, RunData<seq_work>&, resource_type, Args... args)
+  using B = std::decay_t<Body>;
+  using list_type = typename dispatch_list<Dispatch>::type;
+  static_assert(!ON_DEVICE || std::is_trivially_copyable<B>::value,
+                "GPU WorkGroup loop bodies are copied bytewise: capture values and pointers");
+  const int64_t len = (int64_t)seg.size();
+  if (len <= 0) return;
+  uint32_t type_index = 0;
+  if (!std::is_same<list_type, void>::value)
   {
-    const size_t n = s.size();
-    for (size_t k = 0; k < n; ++k)
-    {
-      const size_t l = std::is_same<Order, reverse_ordered>::value ? n - 1 - k : k;
-      const LoopRecord& r = s.loops[l];
-      for (int64_t j = 0; j < r.len; ++j)
-      {
-        s.host_fns[l].call(s.body(l), Index(r.begin + j), args...);
-      }
-    }
+    constexpr int k = index_in_list<
+        B, std::conditional_t<std::is_same<list_type, void>::value, camp::list<>, list_type>>::value;
+    static_assert(std::is_same<list_type, void>::value || k >= 0,
+                  "direct_dispatch: body type is not in the dispatch list");
+    type_index = (uint32_t)(k < 0 ? 0 : k);
   }
-};
+  storage.template add<ON_DEVICE>((int64_t)*seg.begin(), len, type_index, std::forward<Body>(body));
+}
 
-template<typename Order, typename Dispatch, typename Index, typename... Args>
-struct WorkRunner<omp_work, Order, Dispatch, Index, Args...>
+// Host dispatch: the stored call routine of each loop's body type.
+template<typename Index, typename... Args>
+struct HostDispatcher
 {
-  using resource_type = resources::Host;
   template<typename Storage>
-  static void run(Storage& s, RunData<omp_work>&, resource_type, Args... args)
+  static void call(const Storage& s, size_t l, Index i, Args... args)
   {
-    const size_t n = s.size();
-    for (size_t k = 0; k < n; ++k)
-    {
-      const size_t l = std::is_same<Order, reverse_ordered>::value ? n - 1 - k : k;
-      const LoopRecord& r = s.loops[l];
-#pragma omp parallel for schedule(static)
-      for (int64_t j = 0; j < r.len; ++j)
-      {
-        s.host_fns[l].call(s.body(l), Index(r.begin + j), args...);
-      }
-    }
+    s.host_fns[l].call(s.body(l), i, args...);
   }
 };
 
 }  // namespace workgroup
+
+// The runner of a WorkGroup, specialised per exec policy, with the
+// interface of the reference's detail::WorkRunner
+// (include/RAJA/pattern/WorkGroup/WorkRunner.hpp:112-330,
+// include/RAJA/policy/hip/WorkGroup/WorkRunner.hpp:283-387):
+//   enqueue(storage, iterable, body)      record a loop (at WorkPool::enqueue);
+//   run(storage, resource, args...) -> per_run_storage
+//                                         run every loop (at WorkGroup::run;
+//                                         the WorkSite keeps what it returns);
+//   clear()                               drop state before reuse or destruction;
+// plus dispatcher_type and resource_type.
+template<typename EXEC_POLICY_T, typename ORDER_POLICY_T, typename DISPATCH_POLICY_T,
+         typename ALLOCATOR_T, typename INDEX_T, typename... Args>
+struct WorkRunner;
+
+// Host runners: loops in (reverse) enqueue order, or any order for the
+// unordered policies, which run them in enqueue order too.
+template<typename ExecPolicy, typename Order, typename Dispatch, typename Alloc, typename Index,
+         typename... Args>
+struct HostWorkRunner
+{
+  using resource_type = resources::Host;
+  using dispatcher_type = workgroup::HostDispatcher<Index, Args...>;
+  using per_run_storage = int;
+
+  template<typename WorkContainer, typename Iterable, typename LoopBody>
+  void enqueue(WorkContainer& storage, Iterable&& iter, LoopBody&& loop_body)
+  {
+    workgroup::enqueue_loop<false, Dispatch>(storage, std::forward<Iterable>(iter),
+                                             std::forward<LoopBody>(loop_body));
+  }
+
+  template<typename WorkContainer>
+  per_run_storage run(WorkContainer const& s, resource_type, Args... args) const
+  {
+    const size_t n = s.size();
+    for (size_t k = 0; k < n; ++k)
+    {
+      const size_t l = std::is_same<Order, reverse_ordered>::value ? n - 1 - k : k;
+      const workgroup::LoopRecord& r = s.loops[l];
+      if (std::is_same<ExecPolicy, omp_work>::value)
+      {
+#if defined(RAJA_ENABLE_OPENMP)
+#pragma omp parallel for schedule(static)
+#endif
+        for (int64_t j = 0; j < r.len; ++j)
+        {
+          dispatcher_type::call(s, l, Index(r.begin + j), args...);
+        }
+      }
+      else
+      {
+        for (int64_t j = 0; j < r.len; ++j)
+        {
+          dispatcher_type::call(s, l, Index(r.begin + j), args...);
+        }
+      }
+    }
+    return 0;
+  }
+
+  void clear() {}
+};
+
+template<typename Order, typename Dispatch, typename Alloc, typename Index, typename... Args>
+struct WorkRunner<seq_work, Order, Dispatch, Alloc, Index, Args...>
+    : HostWorkRunner<seq_work, Order, Dispatch, Alloc, Index, Args...>
+{
+};
+
+template<typename Order, typename Dispatch, typename Alloc, typename Index, typename... Args>
+struct WorkRunner<omp_work, Order, Dispatch, Alloc, Index, Args...>
+    : HostWorkRunner<omp_work, Order, Dispatch, Alloc, Index, Args...>
+{
+};
+
 }  // namespace detail
 
 template<typename WORKGROUP_POLICY_T, typename INDEX_T, typename EXTRA_ARGS_T,
@@ -340,16 +389,18 @@ template<typename E, typename O, typename S, typename D, typename INDEX_T, typen
          typename ALLOCATOR_T>
 struct WorkSite<WorkGroupPolicy<E, O, S, D>, INDEX_T, xargs<Args...>, ALLOCATOR_T>
 {
-  using runner = detail::workgroup::WorkRunner<E, O, D, INDEX_T, Args...>;
+  using runner = detail::WorkRunner<E, O, D, ALLOCATOR_T, INDEX_T, Args...>;
   using resource_type = typename runner::resource_type;
-  explicit WorkSite(resource_type r) : m_resource(r) {}
+  using per_run_storage = typename runner::per_run_storage;
+  WorkSite(resource_type r, per_run_storage&& prs) : m_resource(r), m_run_storage(std::move(prs)) {}
   WorkSite(WorkSite&&) = default;
   WorkSite& operator=(WorkSite&&) = default;
   resource_type get_resource() const { return m_resource; }
-  void clear() {}
+  void clear() { m_run_storage = per_run_storage(); }
 
 private:
   resource_type m_resource;
+  per_run_storage m_run_storage;
 };
 
 template<typename E, typename O, typename S, typename D, typename INDEX_T, typename... Args,
@@ -359,35 +410,34 @@ struct WorkGroup<WorkGroupPolicy<E, O, S, D>, INDEX_T, xargs<Args...>, ALLOCATOR
   using policy = WorkGroupPolicy<E, O, S, D>;
   using index_type = INDEX_T;
   using storage_type = detail::workgroup::LoopStorage<INDEX_T, Args...>;
-  using runner = detail::workgroup::WorkRunner<E, O, D, INDEX_T, Args...>;
-  using resource_type = typename runner::resource_type;
+  using workrunner_type = detail::WorkRunner<E, O, D, ALLOCATOR_T, INDEX_T, Args...>;
+  using resource_type = typename workrunner_type::resource_type;
   using worksite_type = WorkSite<policy, INDEX_T, xargs<Args...>, ALLOCATOR_T>;
 
   WorkGroup(WorkGroup&&) = default;
   WorkGroup& operator=(WorkGroup&&) = default;
   WorkGroup(const WorkGroup&) = delete;
-  ~WorkGroup() { m_run.release(); }
+  ~WorkGroup() { m_runner.clear(); }
 
   worksite_type run(resource_type r, Args... args)
   {
-    if (m_storage.size() > 0) runner::run(m_storage, m_run, r, args...);
-    return worksite_type(r);
+    return worksite_type(r, m_runner.run(m_storage, r, args...));
   }
   worksite_type run(Args... args) { return run(resource_type::get_default(), args...); }
 
   size_t num_loops() const { return m_storage.size(); }
   void clear()
   {
-    m_run.release();
+    m_runner.clear();
     m_storage.clear();
   }
 
 private:
   template<typename, typename, typename, typename>
   friend struct WorkPool;
-  explicit WorkGroup(storage_type&& s) : m_storage(std::move(s)) {}
+  WorkGroup(storage_type&& s, workrunner_type&& r) : m_storage(std::move(s)), m_runner(std::move(r)) {}
   storage_type m_storage;
-  detail::workgroup::RunData<E> m_run;
+  workrunner_type m_runner;
 };
 
 template<typename E, typename O, typename S, typename D, typename INDEX_T, typename... Args,
@@ -398,7 +448,7 @@ struct WorkPool<WorkGroupPolicy<E, O, S, D>, INDEX_T, xargs<Args...>, ALLOCATOR_
   using index_type = INDEX_T;
   using workgroup_type = WorkGroup<policy, INDEX_T, xargs<Args...>, ALLOCATOR_T>;
   using storage_type = typename workgroup_type::storage_type;
-  using list_type = typename detail::workgroup::dispatch_list<D>::type;
+  using workrunner_type = typename workgroup_type::workrunner_type;
 
   explicit WorkPool(const ALLOCATOR_T& a = ALLOCATOR_T()) : m_alloc(a) {}
   WorkPool(const WorkPool&) = delete;
@@ -414,40 +464,30 @@ struct WorkPool<WorkGroupPolicy<E, O, S, D>, INDEX_T, xargs<Args...>, ALLOCATOR_
   template<typename Segment, typename Body>
   void enqueue(Segment&& seg, Body&& body)
   {
-    using B = std::decay_t<Body>;
-    static_assert(!detail::workgroup::copies_bodies_to_device<E>::value ||
-                      std::is_trivially_copyable<B>::value,
-                  "GPU WorkGroup loop bodies are copied bytewise: capture values and pointers");
-    const int64_t len = (int64_t)seg.size();
-    if (len <= 0) return;  // empty segments are ignored
-    uint32_t type_index = 0;
-    if (!std::is_same<list_type, void>::value)
-    {
-      constexpr int k = detail::workgroup::index_in_list<
-          B, std::conditional_t<std::is_same<list_type, void>::value, camp::list<>, list_type>>::value;
-      static_assert(std::is_same<list_type, void>::value || k >= 0,
-                    "direct_dispatch: body type is not in the dispatch list");
-      type_index = (uint32_t)(k < 0 ? 0 : k);
-    }
-    m_storage.template add<detail::workgroup::copies_bodies_to_device<E>::value>(
-        (int64_t)*seg.begin(), len, type_index, std::forward<Body>(body));
+    if (m_storage.size() == 0) reserve(m_max_loops, m_max_bytes);  // auto-reserve on reuse
+    m_runner.enqueue(m_storage, std::forward<Segment>(seg), std::forward<Body>(body));
     m_max_loops = std::max(m_max_loops, m_storage.size());
     m_max_bytes = std::max(m_max_bytes, m_storage.used);
   }
 
   workgroup_type instantiate()
   {
-    workgroup_type g(std::move(m_storage));
+    workgroup_type g(std::move(m_storage), std::move(m_runner));
     m_storage = storage_type();
-    m_storage.reserve(m_max_loops, m_max_bytes);  // auto-reserve the previous maxima
+    m_runner = workrunner_type();
     return g;
   }
 
-  void clear() { m_storage.clear(); }
+  void clear()
+  {
+    m_runner.clear();
+    m_storage.clear();
+  }
 
 private:
   ALLOCATOR_T m_alloc;
   storage_type m_storage;
+  workrunner_type m_runner;
   size_t m_max_loops = 0;
   size_t m_max_bytes = 0;
 };

@@ -1,9 +1,11 @@
 // RAJA::forall front-end (include/RAJA/pattern/forall.hpp:545-627 in the
 // reference): forall<ExecPolicy>([resource,] iterable, params..., body).
 // The trailing arguments before the body are expt::Reduce / ReduceLoc /
-// Name parameters; the body receives one ValOp& per reducer after the index.
-// Dispatch is by overload on the policy type (forall_impl in each policy's
-// namespace, found by argument-dependent lookup).
+// Name parameters, gathered into an expt::ForallParamPack that each
+// backend's forall_impl takes by value (include/RAJA/policy/hip/forall.hpp:
+// 543-560 in the reference); the body receives one ValOp& per reducer after
+// the index.  Dispatch is by overload on the policy type (forall_impl in each
+// policy's namespace, found by argument-dependent lookup).
 #pragma once
 
 #include <type_traits>
@@ -25,14 +27,15 @@ template<typename ExecPolicy, typename Res, typename Iterable, typename... Args,
 resources::EventProxy<std::decay_t<Res>> forall(Res r, Iterable&& it, Args&&... args)
 {
   static_assert(sizeof...(Args) >= 1, "forall needs a loop body");
-  auto pack = detail::make_forall_param_pack(std::forward<Args>(args)...);
-  auto&& body = detail::last_arg(std::forward<Args>(args)...);
-  util::PluginScope tools(util::make_context<ExecPolicy>(pack.name ? pack.name : ""));
+  auto f_params = expt::make_forall_param_pack(std::forward<Args>(args)...);
+  auto&& body = expt::get_lambda(std::forward<Args>(args)...);
+  const char* name = expt::get_kernel_name(f_params);
+  util::PluginScope tools(util::make_context<ExecPolicy>(name ? name : ""));
   if (it.size() <= 0)
   {
     return resources::EventProxy<std::decay_t<Res>>(r);  // empty range: no launch
   }
-  return forall_impl(r, ExecPolicy{}, std::forward<Iterable>(it), body, pack);
+  return forall_impl(r, ExecPolicy{}, std::forward<Iterable>(it), body, std::move(f_params));
 }
 
 template<typename ExecPolicy, typename Iterable, typename... Args,
@@ -58,8 +61,7 @@ void forall_segments(Res r, seq_segit, const TypedIndexSet<Segs...>& iset, Body&
   for (size_t k = 0; k < iset.getNumSegments(); ++k)
   {
     iset.segmentCall(k, [&](const auto& seg) {
-      auto pack = detail::make_forall_param_pack(body);
-      if (seg.size() > 0) forall_impl(r, Each{}, seg, body, pack);
+      if (seg.size() > 0) forall_impl(r, Each{}, seg, body, expt::make_forall_param_pack(body));
     });
   }
   if (detail::segit_exec<SegExec>::wait_after) r.wait();
@@ -77,8 +79,7 @@ void forall_segments(Res r, omp_parallel_segit, const TypedIndexSet<Segs...>& is
   for (long k = 0; k < ns; ++k)
   {
     iset.segmentCall((size_t)k, [&](const auto& seg) {
-      auto pack = detail::make_forall_param_pack(body);
-      if (seg.size() > 0) forall_impl(r, SegExec{}, seg, body, pack);
+      if (seg.size() > 0) forall_impl(r, SegExec{}, seg, body, expt::make_forall_param_pack(body));
     });
   }
 }

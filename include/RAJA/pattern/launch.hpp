@@ -147,21 +147,6 @@ struct TileTCountExecute;
 
 namespace detail
 {
-// body(ctx, valops...)
-template<typename Body, typename VTuple, size_t... I>
-RAJA_HOST_DEVICE RAJA_INLINE void invoke_launch_with(Body& body, LaunchContext& ctx, VTuple& v,
-                                                     std::index_sequence<I...>)
-{
-  body(ctx, std::get<I>(v)...);
-}
-
-template<typename Body, typename... Vs>
-RAJA_HOST_DEVICE RAJA_INLINE void invoke_launch_body(Body& body, LaunchContext& ctx,
-                                                     std::tuple<Vs...>& v)
-{
-  invoke_launch_with(body, ctx, v, std::index_sequence_for<Vs...>{});
-}
-
 template<typename Pol, typename Res>
 struct policy_for_resource;
 template<typename Pol>
@@ -185,20 +170,20 @@ struct policy_for_resource<Pol, resources::Cdna4>
 template<>
 struct LaunchExecute<seq_launch_t>
 {
-  template<typename Body, typename Pack>
+  template<typename Body, typename ForallParam>
   static resources::EventProxy<resources::Host> exec(resources::Host res, const LaunchParams& p,
-                                                     Body&& body, Pack& pack)
+                                                     Body&& body, ForallParam& f_params)
   {
     std::unique_ptr<char[]> shmem(new char[p.shared_mem_size > 0 ? p.shared_mem_size : 1]);
     LaunchContext ctx;
     ctx.shared_mem_ptr = shmem.get();
-    auto vops = RAJA::detail::make_valops(pack.reducers);
+    const seq_launch_t pol{};
+    expt::ParamMultiplexer::parampack_init(pol, f_params);
     {
       std::decay_t<Body> b(body);
-      RAJA::detail::invoke_launch_body(b, ctx, vops);
+      expt::invoke_body(f_params, b, ctx);
     }
-    RAJA::detail::resolve_host_params(
-        pack.reducers, vops, std::make_index_sequence<std::tuple_size<decltype(vops)>::value>{});
+    expt::ParamMultiplexer::parampack_resolve(pol, f_params);
     return resources::EventProxy<resources::Host>(res);
   }
 };
@@ -259,10 +244,11 @@ resources::EventProxy<std::decay_t<Res>> launch(Res res, const LaunchParams& par
 {
   static_assert(sizeof...(Args) >= 1, "launch needs a body");
   using Pol = typename detail::policy_for_resource<POLICY_LIST, std::decay_t<Res>>::type;
-  auto pack = detail::make_forall_param_pack(std::forward<Args>(args)...);
-  auto&& body = detail::last_arg(std::forward<Args>(args)...);
-  util::PluginScope tools(util::make_context<Pol>(pack.name ? pack.name : ""));
-  return LaunchExecute<Pol>::exec(res, params, body, pack);
+  auto f_params = expt::make_forall_param_pack(std::forward<Args>(args)...);
+  auto&& body = expt::get_lambda(std::forward<Args>(args)...);
+  const char* name = expt::get_kernel_name(f_params);
+  util::PluginScope tools(util::make_context<Pol>(name ? name : ""));
+  return LaunchExecute<Pol>::exec(res, params, body, f_params);
 }
 
 // Launch where a run-time place picks the policy.
@@ -299,10 +285,11 @@ auto launch(const LaunchParams& params, Args&&... args)
 {
   using Pol = typename POLICY_LIST::host_policy_t;
   using Res = typename Pol::resource_type;
-  auto pack = detail::make_forall_param_pack(std::forward<Args>(args)...);
-  auto&& body = detail::last_arg(std::forward<Args>(args)...);
-  util::PluginScope tools(util::make_context<Pol>(pack.name ? pack.name : ""));
-  return LaunchExecute<Pol>::exec(Res::get_default(), params, body, pack);
+  auto f_params = expt::make_forall_param_pack(std::forward<Args>(args)...);
+  auto&& body = expt::get_lambda(std::forward<Args>(args)...);
+  const char* name = expt::get_kernel_name(f_params);
+  util::PluginScope tools(util::make_context<Pol>(name ? name : ""));
+  return LaunchExecute<Pol>::exec(Res::get_default(), params, body, f_params);
 }
 
 // ---- loops inside a launch -----------------------------------------------------

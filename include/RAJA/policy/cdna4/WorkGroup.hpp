@@ -58,11 +58,6 @@ namespace detail
 namespace workgroup
 {
 
-template<int B, bool A>
-struct copies_bodies_to_device<policy::cdna4::cdna4_work_t<B, A>> : std::true_type
-{
-};
-
 template<typename Body, typename Index, typename... Args>
 __device__ void device_call(const void* body, Index i, Args... args)
 {
@@ -176,8 +171,11 @@ __global__ __launch_bounds__(B) void loop_kernel(const LoopRecord* __restrict__ 
   }
 }
 
-template<int B, bool A>
-struct RunData<policy::cdna4::cdna4_work_t<B, A>>
+// Device copies of a group's loop bodies and records and, for unordered
+// runs, its tile table: built at the group's first run, kept for later
+// runs, released when the loop set changes or the group goes away.
+template<int B>
+struct DeviceTables
 {
   unsigned char* d_bytes = nullptr;
   LoopRecord* d_loops = nullptr;
@@ -186,9 +184,11 @@ struct RunData<policy::cdna4::cdna4_work_t<B, A>>
   bool built = false;
   hipStream_t stream = nullptr;
 
-  RunData() = default;
-  RunData(RunData&& o) noexcept { *this = std::move(o); }
-  RunData& operator=(RunData&& o) noexcept
+  DeviceTables() = default;
+  DeviceTables(const DeviceTables&) = delete;
+  DeviceTables& operator=(const DeviceTables&) = delete;
+  DeviceTables(DeviceTables&& o) noexcept { *this = std::move(o); }
+  DeviceTables& operator=(DeviceTables&& o) noexcept
   {
     std::swap(d_bytes, o.d_bytes);
     std::swap(d_loops, o.d_loops);
@@ -198,6 +198,7 @@ struct RunData<policy::cdna4::cdna4_work_t<B, A>>
     std::swap(stream, o.stream);
     return *this;
   }
+  ~DeviceTables() { release(); }
 
   void release()
   {
@@ -209,12 +210,13 @@ struct RunData<policy::cdna4::cdna4_work_t<B, A>>
     d_bytes = nullptr;
     d_loops = nullptr;
     d_tiles = nullptr;
+    ntiles = 0;
     built = false;
   }
 
   // Upload the loop bodies, records and (for unordered runs) the tile table.
   template<typename Storage>
-  void build(Storage& s, resources::Cdna4 r, bool tiles, bool indirect)
+  void build(const Storage& s, resources::Cdna4 r, bool tiles, bool indirect)
   {
     stream = r.get_stream();
     std::vector<LoopRecord> recs(s.loops);
@@ -248,26 +250,47 @@ struct RunData<policy::cdna4::cdna4_work_t<B, A>>
   }
 };
 
-template<int B, bool A, typename Order, typename Dispatch, typename Index, typename... Args>
-struct WorkRunner<policy::cdna4::cdna4_work_t<B, A>, Order, Dispatch, Index, Args...>
+}  // namespace workgroup
+
+template<int B, bool A, typename Order, typename Dispatch, typename Alloc, typename Index,
+         typename... Args>
+struct WorkRunner<policy::cdna4::cdna4_work_t<B, A>, Order, Dispatch, Alloc, Index, Args...>
 {
   using resource_type = resources::Cdna4;
+  using dispatcher_type = workgroup::Dispatcher<Dispatch, Index, Args...>;
+  // Nothing is allocated per run: the device tables belong to the group.
+  using per_run_storage = int;
   static constexpr bool unordered = !std::is_same<Order, ordered>::value &&
                                     !std::is_same<Order, reverse_ordered>::value;
-  static constexpr bool indirect = std::is_same<typename dispatch_list<Dispatch>::type, void>::value;
+  static constexpr bool indirect =
+      std::is_same<typename workgroup::dispatch_list<Dispatch>::type, void>::value;
 
-  template<typename Storage>
-  static void run(Storage& s, RunData<policy::cdna4::cdna4_work_t<B, A>>& rd, resource_type r,
-                  Args... args)
+  WorkRunner() = default;
+  WorkRunner(WorkRunner&&) = default;
+  WorkRunner& operator=(WorkRunner&&) = default;
+
+  // Bodies are copied bytewise to the device at the first run.
+  template<typename WorkContainer, typename Iterable, typename LoopBody>
+  void enqueue(WorkContainer& storage, Iterable&& iter, LoopBody&& loop_body)
   {
-    if (!rd.built) rd.build(s, r, unordered, indirect);
+    m_tables.release();  // the loop set changes: tables are rebuilt at the next run
+    workgroup::enqueue_loop<true, Dispatch>(storage, std::forward<Iterable>(iter),
+                                            std::forward<LoopBody>(loop_body));
+  }
+
+  template<typename WorkContainer>
+  per_run_storage run(WorkContainer const& s, resource_type r, Args... args) const
+  {
+    if (s.size() == 0) return 0;
+    if (!m_tables.built) m_tables.build(s, r, unordered, indirect);
     hipStream_t st = r.get_stream();
     if (unordered)
     {
-      if (rd.ntiles > 0)
+      if (m_tables.ntiles > 0)
       {
-        hipLaunchKernelGGL((tiles_kernel<B, Dispatch, Index, Args...>), dim3((unsigned)rd.ntiles),
-                           dim3(B), 0, st, rd.d_tiles, rd.d_loops, rd.d_bytes, args...);
+        hipLaunchKernelGGL((workgroup::tiles_kernel<B, Dispatch, Index, Args...>),
+                           dim3((unsigned)m_tables.ntiles), dim3(B), 0, st, m_tables.d_tiles,
+                           m_tables.d_loops, m_tables.d_bytes, args...);
         resources::detail::check_hip(hipGetLastError(), "workgroup tiles launch");
       }
     }
@@ -276,7 +299,7 @@ struct WorkRunner<policy::cdna4::cdna4_work_t<B, A>, Order, Dispatch, Index, Arg
       static int cap = 0;
       if (cap == 0)
       {
-        cap = RAJA::cdna4::detail::occupancy_grid(loop_kernel<B, Dispatch, Index, Args...>, B);
+        cap = RAJA::cdna4::detail::occupancy_grid(workgroup::loop_kernel<B, Dispatch, Index, Args...>, B);
       }
       const size_t n = s.size();
       for (size_t k = 0; k < n; ++k)
@@ -284,15 +307,20 @@ struct WorkRunner<policy::cdna4::cdna4_work_t<B, A>, Order, Dispatch, Index, Arg
         const size_t l = std::is_same<Order, reverse_ordered>::value ? n - 1 - k : k;
         int64_t grid = (s.loops[l].len + B - 1) / B;
         if (grid > cap) grid = cap;
-        hipLaunchKernelGGL((loop_kernel<B, Dispatch, Index, Args...>), dim3((unsigned)grid), dim3(B),
-                           0, st, rd.d_loops, (uint32_t)l, rd.d_bytes, args...);
+        hipLaunchKernelGGL((workgroup::loop_kernel<B, Dispatch, Index, Args...>), dim3((unsigned)grid),
+                           dim3(B), 0, st, m_tables.d_loops, (uint32_t)l, m_tables.d_bytes, args...);
       }
       resources::detail::check_hip(hipGetLastError(), "workgroup loop launch");
     }
     if (!A) r.wait();
+    return 0;
   }
+
+  void clear() { m_tables.release(); }
+
+private:
+  mutable workgroup::DeviceTables<B> m_tables;
 };
 
-}  // namespace workgroup
 }  // namespace detail
 }  // namespace RAJA

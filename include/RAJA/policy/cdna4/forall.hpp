@@ -92,6 +92,7 @@ struct LaunchInfo
   bool setup = false;
   int nblocks = 0;
   hipStream_t stream = nullptr;
+  int block_threads = 0;  // threads per block of the launch
 };
 
 inline LaunchInfo& launch_info()
@@ -300,52 +301,6 @@ private:
 namespace detail
 {
 
-// ---- expt parameter reductions --------------------------------------------
-
-template<typename... Ps, size_t... I>
-void setup_params(std::tuple<Ps...>& ps, int nblocks, std::index_sequence<I...>)
-{
-  (void)std::initializer_list<int>{
-      0, ((std::get<I>(ps).device_partials =
-               acquire_slots<typename Ps::value_type>(nblocks),
-           std::get<I>(ps).nblocks = nblocks),
-          0)...};
-}
-
-template<typename P, typename V>
-__device__ inline void finalize_one(P& p, V& v)
-{
-  using T = typename P::value_type;
-  using Tree = typename expt::detail::valop_tree<typename P::valop_type>::type;
-  store_block_value<T, Tree>(v.val, static_cast<T*>(p.device_partials));
-}
-
-template<typename... Ps, typename... Vs, size_t... I>
-__device__ inline void finalize_params(std::tuple<Ps...>& ps, std::tuple<Vs...>& vs,
-                                       std::index_sequence<I...>)
-{
-  (void)std::initializer_list<int>{0, (finalize_one(std::get<I>(ps), std::get<I>(vs)), 0)...};
-}
-
-template<typename P>
-void resolve_one(P& p)
-{
-  using T = typename P::value_type;
-  using Tree = typename expt::detail::valop_tree<typename P::valop_type>::type;
-  const T* slots = static_cast<const T*>(p.device_partials);
-  T r = typename P::valop_type().val;
-  for (int b = 0; b < p.nblocks; ++b) r = Tree{}(r, slots[b]);
-  expt::detail::combine_target(p, r);
-  release_slots(p.device_partials);
-  p.device_partials = nullptr;
-}
-
-template<typename... Ps, size_t... I>
-void resolve_params(std::tuple<Ps...>& ps, std::index_sequence<I...>)
-{
-  (void)std::initializer_list<int>{0, (resolve_one(std::get<I>(ps)), 0)...};
-}
-
 // ---- the forall kernel -------------------------------------------------------
 
 constexpr int kForallItems = 6;  // indices per thread per tile
@@ -360,12 +315,11 @@ inline Index_type forall_grid_cap(int block)
   return (Index_type)raja_cdna4_num_cus() * (Index_type)std::max(1, 1024 / block);
 }
 
-template<int BLOCK, typename Seg, typename Body, typename Reducers>
-__global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, Reducers reducers,
+template<typename EXEC_POL, int BLOCK, typename Seg, typename Body, typename ForallParam>
+__global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, ForallParam f_params,
                                                        Index_type len)
 {
   Body b(body);  // device-private copy: its reducers fold per block at scope end
-  auto vops = RAJA::detail::make_valops(reducers);
   constexpr int U = kForallItems;
   const Index_type tstride = (Index_type)gridDim.x * BLOCK * U;
   for (Index_type base = (Index_type)blockIdx.x * BLOCK * U; base < len; base += tstride)
@@ -373,19 +327,21 @@ __global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, Reduc
     if (base + BLOCK * U <= len)
     {
 #pragma unroll
-      for (int u = 0; u < U; ++u) RAJA::detail::invoke_body(b, seg.at(base + u * BLOCK + threadIdx.x), vops);
+      for (int u = 0; u < U; ++u) RAJA::expt::invoke_body(f_params, b, seg.at(base + u * BLOCK + threadIdx.x));
     }
     else
     {
       for (int u = 0; u < U; ++u)
       {
         const Index_type k = base + u * BLOCK + threadIdx.x;
-        if (k < len) RAJA::detail::invoke_body(b, seg.at(k), vops);
+        if (k < len) RAJA::expt::invoke_body(f_params, b, seg.at(k));
       }
     }
   }
-  finalize_params(reducers, vops,
-                  std::make_index_sequence<std::tuple_size<Reducers>::value>{});
+  if constexpr (!RAJA::expt::type_traits::is_ForallParamPack_empty<ForallParam>::value)
+  {
+    RAJA::expt::ParamMultiplexer::parampack_combine(EXEC_POL{}, f_params);
+  }
 }
 
 template<typename Kernel>
@@ -405,6 +361,73 @@ int occupancy_grid(Kernel k, int block)
 }  // namespace cdna4
 
 RAJA_CDNA4_DECLARE_REDUCERS(cdna4_reduce, cdna4::Cdna4ReduceImpl)
+
+namespace cdna4
+{
+namespace detail
+{
+// Launch information handed to param_init / param_resolve (the reference's
+// hipInfo, include/RAJA/policy/hip/MemUtils_HIP.hpp): the grid of the launch
+// whose reducers are being set up and the resource it runs on.
+struct cdna4Info
+{
+  cdna4Info(resources::Cdna4 r, dim3 grid, dim3 block) : gridDim(grid), blockDim(block), res(r) {}
+  dim3 gridDim;
+  dim3 blockDim;
+  resources::Cdna4 res;
+};
+}  // namespace detail
+}  // namespace cdna4
+
+namespace expt
+{
+namespace detail
+{
+// ---- expt parameter reductions on cdna4 policies -----------------------------
+// Counterpart of include/RAJA/policy/hip/params/reduce.hpp:19-60: each
+// block of the launch folds its threads' ValOps with the fixed block tree
+// and stores one value (param_combine, device side, at kernel end); after
+// the launch the host waits and folds the block values in block order, then
+// combines the result into the target (param_resolve).  No device atomics:
+// the result depends only on the range and the launch shape.
+
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+std::enable_if_t<RAJA::type_traits::is_cdna4_policy<EXEC_POL>::value> param_init(
+    EXEC_POL const&, Reducer<OP, T, VOp>& red, const RAJA::cdna4::detail::cdna4Info& info)
+{
+  const int n = (int)(info.gridDim.x * info.gridDim.y * info.gridDim.z);
+  red.block_values = RAJA::cdna4::detail::acquire_slots<T>(n);
+  red.nblocks = n;
+  red.m_valop = VOp{};
+}
+
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+RAJA_HOST_DEVICE std::enable_if_t<RAJA::type_traits::is_cdna4_policy<EXEC_POL>::value> param_combine(
+    EXEC_POL const&, Reducer<OP, T, VOp>& red)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  RAJA::cdna4::detail::store_block_value<T, reducer_tree<Reducer<OP, T, VOp>>>(red.m_valop.val,
+                                                                               red.block_values);
+#else
+  (void)red;
+#endif
+}
+
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+std::enable_if_t<RAJA::type_traits::is_cdna4_policy<EXEC_POL>::value> param_resolve(
+    EXEC_POL const&, Reducer<OP, T, VOp>& red, const RAJA::cdna4::detail::cdna4Info& info)
+{
+  resources::Cdna4 res = info.res;
+  res.wait();
+  T r = VOp{}.val;
+  for (int b = 0; b < red.nblocks; ++b) r = reducer_tree<Reducer<OP, T, VOp>>{}(r, red.block_values[b]);
+  red.combineTarget(r);
+  RAJA::cdna4::detail::release_slots(red.block_values);
+  red.block_values = nullptr;
+  red.nblocks = 0;
+}
+}  // namespace detail
+}  // namespace expt
 
 namespace policy
 {
@@ -429,35 +452,36 @@ namespace policy
 namespace cdna4
 {
 
-template<int B, bool Async, int G, typename Iterable, typename Body, typename Pack>
-resources::EventProxy<resources::Cdna4> forall_impl(resources::Cdna4 res,
-                                                    const cdna4_exec_t<B, Async, G>&,
-                                                    Iterable&& it, Body&& body, Pack& pack)
+template<int B, bool Async, int G, typename Iterable, typename Body, typename ForallParam>
+std::enable_if_t<expt::type_traits::is_ForallParamPack<ForallParam>::value,
+                 resources::EventProxy<resources::Cdna4>>
+forall_impl(resources::Cdna4 res, const cdna4_exec_t<B, Async, G>& pol, Iterable&& it, Body&& body,
+            ForallParam f_params)
 {
+  using EXEC_POL = cdna4_exec_t<B, Async, G>;
   using Seg = std::decay_t<Iterable>;
   using BodyT = std::decay_t<Body>;
-  using Reducers = decltype(pack.reducers);
-  constexpr size_t NP = std::tuple_size<Reducers>::value;
+  constexpr bool has_params = !expt::type_traits::is_ForallParamPack_empty<ForallParam>::value;
   const Index_type len = (Index_type)it.size();
-  auto kernel = RAJA::cdna4::detail::forall_kernel<B, Seg, BodyT, Reducers>;
+  auto kernel = RAJA::cdna4::detail::forall_kernel<EXEC_POL, B, Seg, BodyT, ForallParam>;
   constexpr Index_type per_block = (Index_type)B * RAJA::cdna4::detail::kForallItems;
   Index_type grid = (len + per_block - 1) / per_block;
   const Index_type cap = (G > 0) ? G : RAJA::cdna4::detail::forall_grid_cap(B);
   if (grid > cap) grid = cap;
-  RAJA::cdna4::detail::setup_params(pack.reducers, (int)grid, std::make_index_sequence<NP>{});
+  RAJA::cdna4::detail::cdna4Info info(res, dim3((unsigned)grid), dim3((unsigned)B));
+  if constexpr (has_params) expt::ParamMultiplexer::parampack_init(pol, f_params, info);
   hipStream_t stream = res.get_stream();
   RAJA::cdna4::detail::LaunchInfo& li = RAJA::cdna4::detail::launch_info();
-  li = RAJA::cdna4::detail::LaunchInfo{true, (int)grid, stream};
-  BodyT launch_body(body);  // reducers in the body take their block slots here
+  li = RAJA::cdna4::detail::LaunchInfo{true, (int)grid, stream, B};
+  BodyT launch_body(body);  // old-style reducers in the body take their block slots here
   li.setup = false;
   Seg seg(it);
-  hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(B), 0, stream, seg, launch_body,
-                     pack.reducers, len);
+  hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(B), 0, stream, seg, launch_body, f_params,
+                     len);
   resources::detail::check_hip(hipGetLastError(), "forall launch");
-  if (NP > 0)
+  if constexpr (has_params)
   {
-    res.wait();  // parameter results resolve at the end of forall
-    RAJA::cdna4::detail::resolve_params(pack.reducers, std::make_index_sequence<NP>{});
+    expt::ParamMultiplexer::parampack_resolve(pol, f_params, info);  // waits for the stream
   }
   else if (!Async)
   {

@@ -199,6 +199,27 @@ __device__ __forceinline__ T atomic_add_acq_rel_agent(T* p, T v)
                                 __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bounded waits.  A spin on another block's publication ends after
+// kSpinLimitTicks of the 100 MHz wall clock (about 2.7 s, far beyond any
+// legitimate wait, which lasts at most one predecessor block's run time);
+// the caller then records the failure in a device word the host can query
+// and proceeds, so a broken publication protocol gives a reported error
+// instead of a hung GPU.  The clock is read once per 256 polls.
+constexpr uint64_t kSpinLimitTicks = 1ull << 28;
+
+struct SpinBound
+{
+  uint64_t t0 = 0;
+  uint32_t polls = 0;
+  __device__ __forceinline__ bool expired()
+  {
+    if ((++polls & 255u) != 0u) return false;
+    const uint64_t t = wall_clock64();
+    if (t0 == 0) t0 = t;
+    return t - t0 > kSpinLimitTicks;
+  }
+};
+
 // Per-stream control block for single-pass kernels.  All counters return to
 // zero when a kernel's last block finishes, so consecutive kernels on one
 // stream (and replays of a captured graph) reuse it without any clearing.

@@ -192,16 +192,18 @@ struct launch_bound
   static constexpr int value = NT > 0 ? NT : 1024;
 };
 
-template<int NT, typename Body, typename Reducers>
-__global__ __launch_bounds__(launch_bound<NT>::value) void launch_kernel(Body body, Reducers reducers)
+template<typename Pol, int NT, typename Body, typename ForallParam>
+__global__ __launch_bounds__(launch_bound<NT>::value) void launch_kernel(Body body, ForallParam f_params)
 {
   extern __shared__ __attribute__((aligned(16))) char raja_cdna4_launch_shmem[];
   LaunchContext ctx;
   ctx.shared_mem_ptr = raja_cdna4_launch_shmem;
   Body b(body);  // device-private copy: its old-style reducers fold per team at scope end
-  auto vops = RAJA::detail::make_valops(reducers);
-  RAJA::detail::invoke_launch_body(b, ctx, vops);
-  finalize_params(reducers, vops, std::make_index_sequence<std::tuple_size<Reducers>::value>{});
+  RAJA::expt::invoke_body(f_params, b, ctx);
+  if constexpr (!RAJA::expt::type_traits::is_ForallParamPack_empty<ForallParam>::value)
+  {
+    RAJA::expt::ParamMultiplexer::parampack_combine(Pol{}, f_params);
+  }
 }
 
 }  // namespace detail
@@ -210,13 +212,14 @@ __global__ __launch_bounds__(launch_bound<NT>::value) void launch_kernel(Body bo
 template<bool Async, int NT>
 struct LaunchExecute<policy::cdna4::cdna4_launch_t<Async, NT>>
 {
-  template<typename Body, typename Pack>
+  template<typename Body, typename ForallParam>
   static resources::EventProxy<resources::Cdna4> exec(resources::Cdna4 res, const LaunchParams& p,
-                                                      Body&& body, Pack& pack)
+                                                      Body&& body, ForallParam& f_params)
   {
+    using Pol = policy::cdna4::cdna4_launch_t<Async, NT>;
     using BodyT = std::decay_t<Body>;
-    using Reducers = decltype(pack.reducers);
-    constexpr size_t NP = std::tuple_size<Reducers>::value;
+    constexpr bool has_params = !expt::type_traits::is_ForallParamPack_empty<ForallParam>::value;
+    const Pol pol{};
     const dim3 grid((unsigned)p.teams.value[0], (unsigned)p.teams.value[1], (unsigned)p.teams.value[2]);
     const dim3 block((unsigned)p.threads.value[0], (unsigned)p.threads.value[1],
                      (unsigned)p.threads.value[2]);
@@ -226,8 +229,12 @@ struct LaunchExecute<policy::cdna4::cdna4_launch_t<Async, NT>>
         p.threads.value[0] <= 0 || p.threads.value[1] <= 0 || p.threads.value[2] <= 0)
     {
       // nothing to run: parameters resolve to their initial targets
-      auto vops = RAJA::detail::make_valops(pack.reducers);
-      RAJA::detail::resolve_host_params(pack.reducers, vops, std::make_index_sequence<NP>{});
+      RAJA::cdna4::detail::cdna4Info none(res, dim3(0, 0, 0), dim3(0, 0, 0));
+      if constexpr (has_params)
+      {
+        expt::ParamMultiplexer::parampack_init(pol, f_params, none);
+        expt::ParamMultiplexer::parampack_resolve(pol, f_params, none);
+      }
       return resources::EventProxy<resources::Cdna4>(res);
     }
     if (nthreads > RAJA::cdna4::detail::launch_bound<NT>::value || nteams > 0x7FFFFFFFLL)
@@ -235,18 +242,18 @@ struct LaunchExecute<policy::cdna4::cdna4_launch_t<Async, NT>>
       throw std::runtime_error("RAJA::launch: team size above the policy's thread bound, or too many teams");
     }
     hipStream_t stream = res.get_stream();
-    RAJA::cdna4::detail::setup_params(pack.reducers, (int)nteams, std::make_index_sequence<NP>{});
+    RAJA::cdna4::detail::cdna4Info info(res, grid, block);
+    if constexpr (has_params) expt::ParamMultiplexer::parampack_init(pol, f_params, info);
     RAJA::cdna4::detail::LaunchInfo& li = RAJA::cdna4::detail::launch_info();
-    li = RAJA::cdna4::detail::LaunchInfo{true, (int)nteams, stream};
+    li = RAJA::cdna4::detail::LaunchInfo{true, (int)nteams, stream, (int)nthreads};
     BodyT launch_body(body);  // old-style reducers in the body take their team slots here
     li.setup = false;
-    hipLaunchKernelGGL((RAJA::cdna4::detail::launch_kernel<NT, BodyT, Reducers>), grid, block,
-                       p.shared_mem_size, stream, launch_body, pack.reducers);
+    hipLaunchKernelGGL((RAJA::cdna4::detail::launch_kernel<Pol, NT, BodyT, ForallParam>), grid, block,
+                       p.shared_mem_size, stream, launch_body, f_params);
     resources::detail::check_hip(hipGetLastError(), "launch");
-    if (NP > 0)
+    if constexpr (has_params)
     {
-      res.wait();
-      RAJA::cdna4::detail::resolve_params(pack.reducers, std::make_index_sequence<NP>{});
+      expt::ParamMultiplexer::parampack_resolve(pol, f_params, info);  // waits for the stream
     }
     else if (!Async)
     {

@@ -3,10 +3,11 @@
 // include/RAJA/policy/openmp/forall.hpp:60-120 in the reference:
 //   * seq forall calls the body for every index in order on a copy of the
 //     body, so captured reducers fold back when the copy is destroyed;
-//   * omp forall privatises the body per thread and splits the range with a
-//     static schedule; parameter reductions are folded in thread order after
-//     the region, which makes them deterministic (the reference combines
-//     them in a critical section, in whatever order threads arrive).
+//   * omp forall privatises the body and the parameter pack per thread and
+//     splits the range with a static schedule; the thread copies of the
+//     parameters are folded (param_combine) in thread order after the region,
+//     which makes them deterministic (the reference combines them in a
+//     critical section, in whatever order threads arrive).
 #pragma once
 
 #include <algorithm>
@@ -32,45 +33,71 @@ using OmpReduceImpl = reduce::detail::HostCombinable<T, Op, true>;
 RAJA_CDNA4_DECLARE_REDUCERS(seq_reduce, SeqReduceImpl)
 RAJA_CDNA4_DECLARE_REDUCERS(omp_reduce, OmpReduceImpl)
 
+namespace expt
+{
 namespace detail
 {
-template<typename... Ps, typename... Vs, size_t... I>
-void resolve_host_params(std::tuple<Ps...>& ps, std::tuple<Vs...>& vs, std::index_sequence<I...>)
+// seq_exec / omp_parallel_for_exec (and the sequential launch) parameter
+// reductions (include/RAJA/policy/sequential/params/reduce.hpp:13-47 and
+// include/RAJA/policy/openmp/params/reduce.hpp in the reference).  Thread
+// copies are folded with the deterministic tree operator (ValLoc ties to the
+// lower index) in thread order.
+template<typename P>
+using is_host_param_policy =
+    std::integral_constant<bool, RAJA::type_traits::is_sequential_policy<P>::value ||
+                                     RAJA::type_traits::is_openmp_policy<P>::value>;
+
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+std::enable_if_t<is_host_param_policy<EXEC_POL>::value> param_init(EXEC_POL const&,
+                                                                    Reducer<OP, T, VOp>& red)
 {
-  (void)std::initializer_list<int>{
-      0, (expt::detail::combine_target(std::get<I>(ps), std::get<I>(vs).val), 0)...};
+  red.m_valop = VOp{};
 }
 
-template<typename... Vs, size_t... I>
-void fold_valops(std::tuple<Vs...>& into, const std::tuple<Vs...>& from, std::index_sequence<I...>)
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+std::enable_if_t<is_host_param_policy<EXEC_POL>::value> param_combine(EXEC_POL const&,
+                                                                       Reducer<OP, T, VOp>& out,
+                                                                       const Reducer<OP, T, VOp>& in)
 {
-  (void)std::initializer_list<int>{
-      0, (std::get<I>(into).val = typename expt::detail::valop_tree<Vs>::type{}(
-              std::get<I>(into).val, std::get<I>(from).val),
-          0)...};
+  using R = Reducer<OP, T, VOp>;
+  out.m_valop.val = reducer_tree<R>{}(out.m_valop.val, in.m_valop.val);
+}
+
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+std::enable_if_t<is_host_param_policy<EXEC_POL>::value> param_combine(EXEC_POL const&,
+                                                                       Reducer<OP, T, VOp>&)
+{
+}
+
+template<typename EXEC_POL, typename OP, typename T, typename VOp>
+std::enable_if_t<is_host_param_policy<EXEC_POL>::value> param_resolve(EXEC_POL const&,
+                                                                       Reducer<OP, T, VOp>& red)
+{
+  red.combineTarget(red.m_valop.val);
 }
 }  // namespace detail
+}  // namespace expt
 
 namespace policy
 {
 namespace sequential
 {
 
-template<typename Iterable, typename Body, typename Pack>
-resources::EventProxy<resources::Host> forall_impl(resources::Host r, const seq_exec&,
-                                                   Iterable&& it, Body&& body, Pack& pack)
+template<typename Iterable, typename Body, typename ForallParam>
+std::enable_if_t<expt::type_traits::is_ForallParamPack<ForallParam>::value,
+                 resources::EventProxy<resources::Host>>
+forall_impl(resources::Host r, const seq_exec& pol, Iterable&& it, Body&& body, ForallParam f_params)
 {
   const auto n = it.size();
-  auto vops = RAJA::detail::make_valops(pack.reducers);
+  expt::ParamMultiplexer::parampack_init(pol, f_params);
   {
     std::decay_t<Body> b(body);
     for (decltype(it.size()) k = 0; k < n; ++k)
     {
-      RAJA::detail::invoke_body(b, it.at(k), vops);
+      expt::invoke_body(f_params, b, it.at(k));
     }
   }
-  RAJA::detail::resolve_host_params(
-      pack.reducers, vops, std::make_index_sequence<std::tuple_size<decltype(vops)>::value>{});
+  expt::ParamMultiplexer::parampack_resolve(pol, f_params);
   return resources::EventProxy<resources::Host>(r);
 }
 
@@ -79,39 +106,34 @@ resources::EventProxy<resources::Host> forall_impl(resources::Host r, const seq_
 namespace omp
 {
 
-template<typename Iterable, typename Body, typename Pack>
-resources::EventProxy<resources::Host> forall_impl(resources::Host r, const omp_parallel_for_exec&,
-                                                   Iterable&& it, Body&& body, Pack& pack)
+template<typename Iterable, typename Body, typename ForallParam>
+std::enable_if_t<expt::type_traits::is_ForallParamPack<ForallParam>::value,
+                 resources::EventProxy<resources::Host>>
+forall_impl(resources::Host r, const omp_parallel_for_exec& pol, Iterable&& it, Body&& body,
+            ForallParam f_params)
 {
-  using VT = decltype(RAJA::detail::make_valops(pack.reducers));
   const auto n = it.size();
+  expt::ParamMultiplexer::parampack_init(pol, f_params);
 #if defined(RAJA_ENABLE_OPENMP)
-  std::vector<VT> per_thread(omp_get_max_threads());
+  std::vector<ForallParam> per_thread(omp_get_max_threads(), f_params);
 #pragma omp parallel
   {
     std::decay_t<Body> b(body);
-    VT vops = RAJA::detail::make_valops(pack.reducers);
+    ForallParam& mine = per_thread[omp_get_thread_num()];
 #pragma omp for schedule(static)
     for (decltype(it.size()) k = 0; k < n; ++k)
     {
-      RAJA::detail::invoke_body(b, it.at(k), vops);
+      expt::invoke_body(mine, b, it.at(k));
     }
-    per_thread[omp_get_thread_num()] = vops;
   }
-  VT total = RAJA::detail::make_valops(pack.reducers);
-  for (auto& v : per_thread)
-  {
-    RAJA::detail::fold_valops(total, v, std::make_index_sequence<std::tuple_size<VT>::value>{});
-  }
+  for (auto& t : per_thread) expt::ParamMultiplexer::parampack_combine(pol, f_params, t);
 #else
-  VT total = RAJA::detail::make_valops(pack.reducers);
   {
     std::decay_t<Body> b(body);
-    for (decltype(it.size()) k = 0; k < n; ++k) RAJA::detail::invoke_body(b, it.at(k), total);
+    for (decltype(it.size()) k = 0; k < n; ++k) expt::invoke_body(f_params, b, it.at(k));
   }
 #endif
-  RAJA::detail::resolve_host_params(pack.reducers, total,
-                                    std::make_index_sequence<std::tuple_size<VT>::value>{});
+  expt::ParamMultiplexer::parampack_resolve(pol, f_params);
   return resources::EventProxy<resources::Host>(r);
 }
 

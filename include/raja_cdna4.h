@@ -88,6 +88,16 @@ int raja_cdna4_stream_synchronize(void* stream);
  * destroying a stream that was passed to the library). */
 int raja_cdna4_stream_release(void* stream);
 
+/* Waits for the current device, then reports whether any single-pass
+ * kernel (scan or radix pass) gave up waiting for a predecessor tile's
+ * lookback descriptor since the last call: such a wait is bounded (about 2.7 s
+ * of wall clock) so that a broken descriptor protocol returns an error here
+ * instead of hanging the device; results of the affected calls are invalid.
+ * Clears the record.  No reference counterpart: the reference's scans and
+ * sorts are rocPRIM calls whose errors surface through
+ * CAMP_HIP_API_INVOKE_AND_CHECK (include/RAJA/policy/hip/scan.hpp:72-260). */
+int raja_cdna4_device_status(void);
+
 /* Device and pinned-host allocation through the library's pools; used by the
  * C++ front-end's resources::Cdna4 and reducer storage.  Replaces the
  * allocators and mempools of include/RAJA/policy/hip/MemUtils_HIP.hpp:69-160
@@ -258,7 +268,8 @@ typedef struct
   int max_inflight;    /* launch units in flight, 1..64                              */
   int max_batch;       /* requests per batch, >= 1                                   */
   int64_t batch_limit; /* sum/scan requests with n <= batch_limit join batches;
-                          sorts join when n <= 8192                                  */
+                          sorts join when n <= 8192; at most 2^27 (the batched
+                          sum folds at most 4096 tiles of 32768 doubles)             */
   int64_t arena_bytes; /* device working memory of the server                       */
   int honor_arrivals;  /* 1: a request starts no earlier than its arrival_s;
                           0: every request has arrived at time 0 (saturation)        */

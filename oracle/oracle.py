@@ -229,6 +229,14 @@ def omp_multi_reduce_f64(x, nthreads=None):
     return [v.value for v in r] + [loc.value]
 
 
+def omp_sum_f64(x, nthreads=None):
+    """ReduceSum<omp_reduce, double> over one forall
+    (include/RAJA/policy/openmp/reduce.hpp:46-69)."""
+    nthreads = nthreads or omp_max_threads()
+    return _fn("orc_omp_sum_f64", ctypes.c_double)(_p(x), ctypes.c_int64(x.size),
+                                                  ctypes.c_int(nthreads))
+
+
 def omp_exclusive_i32(x, out, init=0, nthreads=None):
     nthreads = nthreads or omp_max_threads()
     _fn("orc_omp_exclusive_i32")(_p(x), _p(out), ctypes.c_int64(x.size), ctypes.c_int32(init),
@@ -300,4 +308,18 @@ def halo_cycle(vars_, pack_lists, unpack_lists, lengths):
     buf = np.empty(int(lengths.sum()) * nv, np.float64)
     _fn("orc_halo_cycle")(ptrs, ctypes.c_int(nv), _p(pack_lists), _p(unpack_lists), _p(lengths),
                           _p(buf))
+    return vars_
+
+
+def omp_halo_cycle(vars_, pack_lists, unpack_lists, lengths, buf=None, nthreads=None):
+    """The example's omp_parallel_for_exec halo cycle
+    (examples/tut_halo-exchange.cpp:615-700): one parallel loop per
+    (neighbour, variable), packs then unpacks."""
+    nthreads = nthreads or omp_max_threads()
+    nv = len(vars_)
+    ptrs = (ctypes.c_void_p * nv)(*[v.ctypes.data for v in vars_])
+    if buf is None:
+        buf = np.empty(int(lengths.sum()) * nv, np.float64)
+    _fn("orc_omp_halo_cycle")(ptrs, ctypes.c_int(nv), _p(pack_lists), _p(unpack_lists),
+                              _p(lengths), _p(buf), ctypes.c_int(nthreads))
     return vars_

@@ -427,6 +427,25 @@ ORC_API void orc_omp_multi_reduce_f64(const double* x, int64_t n, int nthreads,
   *sum = S; *mn = MN; *mx = MX; *mlv = LV; *mll = LL;
 }
 
+/* ReduceSum<omp_reduce, double> over one forall: each thread's copy starts at
+ * 0 and folds its static chunk in index order; the copies fold into the
+ * parent under a critical section as they are destroyed
+ * (include/RAJA/policy/openmp/reduce.hpp:46-69), so the bits depend on
+ * thread arrival order. */
+ORC_API double orc_omp_sum_f64(const double* x, int64_t n, int nthreads)
+{
+  double S = 0.0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    double s = 0.0;
+#pragma omp for schedule(static) nowait
+    for (int64_t i = 0; i < n; ++i) s += x[i];
+#pragma omp critical
+    S += s;
+  }
+  return S;
+}
+
 /* daxpy y = a*x + y then dot(x,y) and minloc(x) in separate foralls */
 ORC_API void orc_omp_daxpy_dot_minloc(const double* x, double* y, int64_t n,
                                       double a, int nthreads, double* dot,
@@ -623,5 +642,40 @@ ORC_API void orc_halo_cycle(double* const* vars, int nvars,
     for (int v = 0; v < nvars; ++v)
       for (int32_t i = 0; i < lengths[l]; ++i) vars[v][list[i]] = buf[(int64_t)v * lengths[l] + i];
     off += lengths[l];
+  }
+}
+
+/* The example's omp_parallel_for_exec variant (tut_halo-exchange.cpp:615-700):
+ * one parallel forall per (neighbour, variable) loop, pack loops then unpack
+ * loops, same buffer layout as orc_halo_cycle. */
+ORC_API void orc_omp_halo_cycle(double* const* vars, int nvars,
+                                const int32_t* pack_lists, const int32_t* unpack_lists,
+                                const int32_t* lengths, double* buffers, int nthreads)
+{
+  int64_t off = 0;
+  for (int l = 0; l < 26; ++l) {
+    double* buf = buffers + off * nvars;
+    const int32_t* list = pack_lists + off;
+    const int32_t len = lengths[l];
+    for (int v = 0; v < nvars; ++v) {
+      double* b = buf + (int64_t)v * len;
+      const double* var = vars[v];
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+      for (int32_t i = 0; i < len; ++i) b[i] = var[list[i]];
+    }
+    off += len;
+  }
+  off = 0;
+  for (int l = 0; l < 26; ++l) {
+    const double* buf = buffers + off * nvars;
+    const int32_t* list = unpack_lists + off;
+    const int32_t len = lengths[l];
+    for (int v = 0; v < nvars; ++v) {
+      const double* b = buf + (int64_t)v * len;
+      double* var = vars[v];
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+      for (int32_t i = 0; i < len; ++i) var[list[i]] = b[i];
+    }
+    off += len;
   }
 }

@@ -88,6 +88,7 @@ def _declare(L):
         "raja_cdna4_num_cus": [],
         "raja_cdna4_stream_synchronize": [vp],
         "raja_cdna4_stream_release": [vp],
+        "raja_cdna4_device_status": [],
         "raja_cdna4_fill": [i32, vp, i64, u64, dbl, dbl, vp],
         "raja_cdna4_reduce": [i32, i32, vp, i64, vp, vp, vp],
         "raja_cdna4_reduce_loc": [i32, i32, vp, i64, vp, i64, vp, vp, vp],
@@ -162,11 +163,44 @@ def _prep(*tensors):
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
+def device_status():
+    """Raise if a scan or radix pass gave up waiting on a lookback descriptor
+    since the last check (waits for the device first)."""
+    _check(lib().raja_cdna4_device_status())
+
+
 def _dt(t):
     try:
         return _DTYPES[t.dtype]
     except KeyError:
         raise RajaError("unsupported dtype %s" % t.dtype)
+
+
+def _same_device(*tensors):
+    """Every tensor (None skipped) on one device; the C entry points take raw
+    pointers and no lengths beyond n, so this and the size checks below run
+    before any call."""
+    devs = {t.device for t in tensors if t is not None}
+    if len(devs) > 1:
+        raise RajaError("tensors on different devices: %s" % sorted(str(d) for d in devs))
+
+
+def _check_out(out, dtype, numel, what="out"):
+    if out.dtype != dtype:
+        raise RajaError("%s has dtype %s, expected %s" % (what, out.dtype, dtype))
+    if out.numel() < numel:
+        raise RajaError("%s holds %d elements, needs at least %d" % (what, out.numel(), numel))
+
+
+def _check_plan(plan, *tensors):
+    """A segment plan's calls read plan.total elements of every span and
+    write plan.nseg results; the spans must be on the plan's device."""
+    if getattr(plan, "handle", None) is None:
+        raise RajaError("segment plan is closed")
+    want = torch.device("cuda", plan.device)
+    for t in tensors:
+        if t is not None and t.device != want:
+            raise RajaError("tensor on %s, plan on %s" % (t.device, want))
 
 
 def _host_scalar(value, dtype):
@@ -188,6 +222,7 @@ _IDENT = {
 def inclusive_scan(in_, out, op=operators.plus):
     if in_.numel() != out.numel() or in_.dtype != out.dtype:
         raise RajaError("in and out spans must match in length and type")
+    _same_device(in_, out)
     s = _prep(in_, out)
     _check(lib().raja_cdna4_inclusive_scan(_dt(in_), op, in_.data_ptr(), out.data_ptr(),
                                            in_.numel(), s))
@@ -197,6 +232,7 @@ def inclusive_scan(in_, out, op=operators.plus):
 def exclusive_scan(in_, out, op=operators.plus, init=None):
     if in_.numel() != out.numel() or in_.dtype != out.dtype:
         raise RajaError("in and out spans must match in length and type")
+    _same_device(in_, out)
     s = _prep(in_, out)
     if init is None:
         init = _IDENT[op](in_.dtype)
@@ -221,6 +257,9 @@ def sort_pairs(keys, vals=None, comparator=operators.less):
         raise RajaError("GPU sorts accept operators.less or operators.greater")
     if vals is not None and vals.numel() != keys.numel():
         raise RajaError("keys and values must have the same length")
+    if vals is not None and vals.element_size() not in (4, 8):
+        raise RajaError("values must be 4- or 8-byte elements")
+    _same_device(keys, vals)
     s = _prep(keys, vals)
     vb = 0 if vals is None else vals.element_size()
     _check(lib().raja_cdna4_sort_pairs(_dt(keys), keys.data_ptr(),
@@ -245,7 +284,10 @@ stable_sort = sort
 
 def reduce(x, op, init=None, out=None):
     """Device result op(init, fold_op(x)) as a 1-element tensor."""
-    s = _prep(x)
+    if out is not None:
+        _check_out(out, x.dtype, 1)
+        _same_device(x, out)
+    s = _prep(x, out)
     if init is None:
         init = {operators.bit_or: 0, operators.bit_and: -1}.get(op, None)
         if init is None:
@@ -294,7 +336,10 @@ def multi_reduce_f64(x, out=None):
     6-slot float64 device tensor (slot 4 holds the int64 loc bits)."""
     if x.dtype != torch.float64:
         raise RajaError("multi_reduce_f64 needs float64")
-    s = _prep(x)
+    if out is not None:
+        _check_out(out, torch.float64, 6)
+        _same_device(x, out)
+    s = _prep(x, out)
     if out is None:
         out = torch.empty(6, dtype=torch.float64, device=x.device)
     _check(lib().raja_cdna4_multi_reduce_f64(x.data_ptr(), x.numel(), out.data_ptr(), s))
@@ -323,6 +368,7 @@ class Segments:
         self.total = int(off[-1])
         h = ctypes.c_void_p()
         _prep()
+        self.device = torch.cuda.current_device()
         _check(lib().raja_cdna4_segments_create(off.ctypes.data_as(ctypes.c_void_p), self.nseg,
                                                 ctypes.byref(h)))
         self.handle = h
@@ -387,18 +433,34 @@ def request(kind, n, in_, out, in_vals=None, out_vals=None, arrival_s=0.0):
 
 
 def segmented_reduce(plan, x, op=operators.plus, init=None, out=None):
-    s = _prep(x)
+    if x.numel() != plan.total:
+        raise RajaError("x holds %d elements, the plan covers %d" % (x.numel(), plan.total))
+    if out is not None:
+        _check_out(out, x.dtype, plan.nseg)
+    _check_plan(plan, x, out)
+    s = _prep(x, out)
     if init is None:
         init = _IDENT[op](x.dtype)
     hold, p = _host_scalar(init, x.dtype)
-    if out is None:
-        out = torch.empty(plan.nseg, dtype=x.dtype, device=x.device)
+    made = out is None
+    if made:
+        out = torch.empty(max(plan.nseg, 1), dtype=x.dtype, device=x.device)
     _check(lib().raja_cdna4_segmented_reduce(plan.handle, _dt(x), op, x.data_ptr(), p,
                                              out.data_ptr(), s))
-    return out
+    return out[:plan.nseg] if made else out
+
+
+def _check_plan_scan(plan, in_, out):
+    if in_.numel() != plan.total or out.numel() != plan.total:
+        raise RajaError("scan spans hold %d and %d elements, the plan covers %d"
+                        % (in_.numel(), out.numel(), plan.total))
+    if in_.dtype != out.dtype:
+        raise RajaError("in and out spans must have the same dtype")
+    _check_plan(plan, in_, out)
 
 
 def segmented_exclusive_scan(plan, in_, out, op=operators.plus, init=None):
+    _check_plan_scan(plan, in_, out)
     s = _prep(in_, out)
     if init is None:
         init = _IDENT[op](in_.dtype)
@@ -409,6 +471,7 @@ def segmented_exclusive_scan(plan, in_, out, op=operators.plus, init=None):
 
 
 def segmented_inclusive_scan(plan, in_, out, op=operators.plus):
+    _check_plan_scan(plan, in_, out)
     s = _prep(in_, out)
     _check(lib().raja_cdna4_segmented_inclusive_scan(plan.handle, _dt(in_), op, in_.data_ptr(),
                                                      out.data_ptr(), s))
@@ -416,6 +479,15 @@ def segmented_inclusive_scan(plan, in_, out, op=operators.plus):
 
 
 def segmented_sort_pairs(plan, keys, vals=None, comparator=operators.less):
+    if comparator not in (operators.less, operators.greater):
+        raise RajaError("GPU sorts accept operators.less or operators.greater")
+    if keys.numel() != plan.total:
+        raise RajaError("keys hold %d elements, the plan covers %d" % (keys.numel(), plan.total))
+    if vals is not None and vals.numel() != keys.numel():
+        raise RajaError("keys and values must have the same length")
+    if vals is not None and vals.element_size() not in (4, 8):
+        raise RajaError("values must be 4- or 8-byte elements")
+    _check_plan(plan, keys, vals)
     s = _prep(keys, vals)
     vb = 0 if vals is None else vals.element_size()
     _check(lib().raja_cdna4_segmented_sort_pairs(
@@ -430,7 +502,15 @@ FILL_U32_BITS, FILL_I32, FILL_I64, FILL_F64_UNIFORM, FILL_F64_NORMAL, FILL_I32_I
     FILL_F64_IOTA, FILL_U32_CONST = range(8)
 
 
+_FILL_BYTES = {0: 4, 1: 4, 2: 8, 3: 8, 4: 8, 5: 4, 6: 8, 7: 4}
+
+
 def fill(kind, out, seed=0, a=0.0, b=0.0):
+    if kind not in _FILL_BYTES:
+        raise RajaError("unknown fill kind %r" % (kind,))
+    if out.element_size() != _FILL_BYTES[kind]:
+        raise RajaError("fill kind %d writes %d-byte elements, out has %d-byte ones"
+                        % (kind, _FILL_BYTES[kind], out.element_size()))
     s = _prep(out)
     _check(lib().raja_cdna4_fill(kind, out.data_ptr(), out.numel(), seed, float(a), float(b), s))
     return out

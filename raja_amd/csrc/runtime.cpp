@@ -231,7 +231,7 @@ bool timing_enabled() { return g_timing; }
 
 TimedScope::TimedScope(const char* n, hipStream_t s) : name(n), stream(s)
 {
-  if (g_timing)
+  if (g_timing && n)  // a null name times nothing (no event is created)
   {
     if (hipEventCreate(&start) == hipSuccess)
     {
@@ -304,6 +304,22 @@ int raja_cdna4_stream_release(void* stream)
   if (c->pinned) RC_HIP(hipHostFree(c->pinned));
   if (c->small) RC_HIP(hipFree(c->small));
   delete c;
+  return RAJA_CDNA4_SUCCESS;
+}
+
+int raja_cdna4_device_status(void)
+{
+  RC_HIP(hipDeviceSynchronize());
+  uint32_t scans = 0, sorts = 0;
+  RC_TRY(rscan::take_spin_timeouts(&scans));
+  RC_TRY(rsort::take_spin_timeouts(&sorts));
+  if (scans || sorts)
+  {
+    return set_error(RAJA_CDNA4_ERROR_HIP,
+                     "%u scan and %u sort lookback waits gave up after %.1f s: the results of "
+                     "those calls are invalid",
+                     scans, sorts, (double)RAJA::cdna4::dev::kSpinLimitTicks / 1e8);
+  }
   return RAJA_CDNA4_SUCCESS;
 }
 

@@ -83,3 +83,14 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace rc
+
+// Counts of lookback waits that gave up (SpinBound) since the last call, per
+// kernel module; each call reads and clears its module's device counter.
+namespace rscan
+{
+int take_spin_timeouts(uint32_t* count);
+}
+namespace rsort
+{
+int take_spin_timeouts(uint32_t* count);
+}

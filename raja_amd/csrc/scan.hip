@@ -143,6 +143,9 @@ __device__ __forceinline__ uint32_t peek(const Desc<T>* d, uint32_t epoch_tag, T
   return st0 & 3u;
 }
 
+// Lookback waits that outlived SpinBound's limit (see raja_cdna4_device_status).
+static __device__ uint32_t g_spin_timeouts;
+
 template<typename T, typename Op>
 __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, int64_t j,
                                       uint32_t epoch_tag)
@@ -153,6 +156,7 @@ __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, int64_t
   const int lane = lane_id();
   T acc = Op::ident();
   int64_t step = 1;
+  SpinBound bound;
   while (true)
   {
     const int64_t back = step + lane;
@@ -166,6 +170,15 @@ __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, int64_t
       {
         __builtin_amdgcn_s_sleep(1);
         if (kind == 0u) kind = peek<T>(desc + t, epoch_tag, v);
+        if (bound.expired())  // the loop and the clock are wave-uniform
+        {
+          if (lane == 0) atomicAdd(&g_spin_timeouts, 1u);
+          if (kind == 0u)
+          {
+            kind = kInc;  // give up on it: the result is reported invalid
+            v = Op::ident();
+          }
+        }
       }
     }
     else
@@ -626,6 +639,19 @@ int seg_scan_dispatch(int dtype, int op, const void* in, void* out, raja_cdna4_s
 
 namespace rscan
 {
+int take_spin_timeouts(uint32_t* count)
+{
+  uint32_t v = 0;
+  RC_HIP(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_spin_timeouts), sizeof(v)));
+  if (v)
+  {
+    const uint32_t zero = 0;
+    RC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_spin_timeouts), &zero, sizeof(zero)));
+  }
+  *count = v;
+  return RAJA_CDNA4_SUCCESS;
+}
+
 int launch_segmented_scan(int dtype, int op, bool exclusive, const void* in, void* out,
                           raja_cdna4_segments_s* plan, const void* init, hipStream_t s)
 {

@@ -352,6 +352,8 @@ namespace rws
 {
 int64_t sum_tile_f64() { return rt::tile_elems<double>(); }
 
+int64_t sum_batch_max() { return (int64_t)kSegMaxGroups * 64 * rt::tile_elems<double>(); }
+
 int table_sum_f64(const double* base, const rs::TileRec* d_tiles, int64_t ntiles,
                   const uint32_t* d_first, const uint32_t* d_count, int64_t nseg,
                   const int64_t* d_out_index, void* ws, hipStream_t s)

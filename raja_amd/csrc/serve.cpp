@@ -513,6 +513,12 @@ int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_s
   {
     return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad server configuration");
   }
+  if (c.batch_limit > rws::sum_batch_max())
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT,
+                         "batch_limit %lld exceeds the batched sum's %lld elements",
+                         (long long)c.batch_limit, (long long)rws::sum_batch_max());
+  }
   raja_cdna4_server_s* sv = new raja_cdna4_server_s();
   sv->cfg = c;
   auto fail = [&](hipError_t e, const char* what) {

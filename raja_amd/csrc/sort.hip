@@ -38,6 +38,9 @@ constexpr uint32_t kCountMask = 0x3FFFFFFFu;
 constexpr int kRowStride = kRadix + 8;
 constexpr int64_t kMaxSortN = (int64_t)kCountMask;  // counts live in 30 bits
 
+// Lookback waits that outlived SpinBound's limit (see raja_cdna4_device_status).
+static __device__ uint32_t g_spin_timeouts;
+
 __global__ void clear_words_kernel(uint32_t* p, int64_t n)
 {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -374,6 +377,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       // descriptor not yet published is re-read.
       const int d = tid;
       bool done = false;
+      SpinBound bound;
       for (int64_t t0 = tile - 1; !done; t0 -= 8)
       {
         uint32_t w[8];
@@ -397,6 +401,11 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
           {
             __builtin_amdgcn_s_sleep(1);
             w[u] = atomic_load_agent(&desc_cur[t * kRowStride + d]);
+            if ((w[u] >> 30) == 0u && bound.expired())
+            {
+              atomicAdd(&g_spin_timeouts, 1u);
+              w[u] = kInc << 30;  // give up on it: the result is reported invalid
+            }
           }
           prefix += w[u] & kCountMask;
           if ((w[u] >> 30) == kInc) done = true;
@@ -654,6 +663,22 @@ int sort_key_type(void* keys, void* vals, int val_bytes, int64_t n, int descendi
 }
 
 }  // namespace
+
+namespace rsort
+{
+int take_spin_timeouts(uint32_t* count)
+{
+  uint32_t v = 0;
+  RC_HIP(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_spin_timeouts), sizeof(v)));
+  if (v)
+  {
+    const uint32_t zero = 0;
+    RC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_spin_timeouts), &zero, sizeof(zero)));
+  }
+  *count = v;
+  return RAJA_CDNA4_SUCCESS;
+}
+}  // namespace rsort
 
 namespace rws
 {

@@ -30,6 +30,9 @@ int sort_pairs_u32_i32(const uint32_t* in_keys, const int32_t* in_vals, uint32_t
 // Sums of nseg FP64 ranges given by tile records (tile = sum_tile_f64());
 // range g's result goes to base[d_out_index[g]].  ws holds ntiles partials.
 int64_t sum_tile_f64();
+// Longest range table_sum_f64 folds: its final fold holds 64 groups of 64
+// tile values.
+int64_t sum_batch_max();
 int table_sum_f64(const double* base, const rs::TileRec* d_tiles, int64_t ntiles,
                   const uint32_t* d_first, const uint32_t* d_count, int64_t nseg,
                   const int64_t* d_out_index, void* ws, hipStream_t s);

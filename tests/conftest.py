@@ -37,3 +37,13 @@ def raja():
     import raja_amd
     raja_amd.lib()
     return raja_amd
+
+
+@pytest.fixture(autouse=True)
+def _lookback_waits_completed(request):
+    """After every test that used the GPU library: no scan or radix pass gave
+    up waiting on a lookback descriptor (such a wait is bounded and recorded
+    instead of hanging the device)."""
+    yield
+    if "raja" in request.fixturenames:
+        request.getfixturevalue("raja").device_status()

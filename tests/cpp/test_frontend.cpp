@@ -1037,6 +1037,120 @@ void test_cdna4_multi()
   }
 }
 
+// Floating-point multi-reductions are deterministic: each wave folds its
+// calls into a private row (same-bin lanes in lane order, calls in program
+// order) and get() folds the rows in wave order.  The host below replays
+// that order for the forall's launch shape (tiles of 6*B indices, a grid of
+// min(tiles, CUs * 1024 / B) blocks), so the device bits must equal it, for
+// two block sizes, and repeats must give the same bits.
+template<int B>
+std::vector<double> emulate_wave_rows(const std::vector<double>& x, int nb, size_t (*bin)(long long))
+{
+  const long long N = (long long)x.size(), U = 6, per_block = (long long)B * U;
+  const long long cap = (long long)raja_cdna4_num_cus() * std::max(1, 1024 / B);
+  const long long grid = std::min((N + per_block - 1) / per_block, cap);
+  const int wpb = B / 64;
+  std::vector<double> rows((size_t)(grid * wpb) * nb, 0.0);
+  for (long long b = 0; b < grid; ++b)
+  {
+    for (int w = 0; w < wpb; ++w)
+    {
+      double* row = &rows[(size_t)(b * wpb + w) * nb];
+      for (long long base = b * per_block; base < N; base += grid * per_block)
+      {
+        for (long long u = 0; u < U; ++u)
+        {
+          bool done[64] = {false};
+          for (int l0 = 0; l0 < 64; ++l0)
+          {
+            const long long i0 = base + u * B + w * 64 + l0;
+            if (i0 >= N || done[l0]) continue;
+            const size_t lb = bin(i0);
+            double r = 0.0;
+            for (int l = l0; l < 64; ++l)
+            {
+              const long long i = base + u * B + w * 64 + l;
+              if (i < N && !done[l] && bin(i) == lb)
+              {
+                r = r + x[(size_t)i];
+                done[l] = true;
+              }
+            }
+            row[lb] = row[lb] + r;
+          }
+        }
+      }
+    }
+  }
+  std::vector<double> out((size_t)nb);
+  for (int k = 0; k < nb; ++k)
+  {
+    double r = 0.0;
+    for (long long q = 0; q < grid * wpb; ++q) r = r + rows[(size_t)q * nb + k];
+    out[(size_t)k] = 0.0 + (0.0 + r);  // host total (identity) then init (identity)
+  }
+  return out;
+}
+
+static size_t bin_spread(long long i) { return (size_t)((i * 919) % 1000); }
+static size_t bin_runs(long long i) { return (size_t)((i / 3) % 7); }
+
+template<int B>
+void check_multi_deterministic(const std::vector<double>& x, const double* dx)
+{
+  const long long N = (long long)x.size();
+  for (int which = 0; which < 2; ++which)
+  {
+    const int nb = which == 0 ? 1000 : 7;
+    std::vector<double> got[2];
+    for (int rep = 0; rep < 2; ++rep)
+    {
+      RAJA::MultiReduceSum<RAJA::cdna4_multi_reduce, double> s(nb);
+      if (which == 0)
+      {
+        RAJA::forall<RAJA::cdna4_exec<B>>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+          s[(i * 919) % 1000] += dx[i];
+        });
+      }
+      else
+      {
+        RAJA::forall<RAJA::cdna4_exec<B>>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+          s[(i / 3) % 7] += dx[i];
+        });
+      }
+      got[rep].resize((size_t)nb);
+      s.get_all(got[rep]);
+    }
+    const auto want = emulate_wave_rows<B>(x, nb, which == 0 ? bin_spread : bin_runs);
+    const bool same = std::memcmp(got[0].data(), got[1].data(), sizeof(double) * nb) == 0;
+    const bool exact = std::memcmp(got[0].data(), want.data(), sizeof(double) * nb) == 0;
+    const std::string t = "cdna4 FP MultiReduceSum B=" + std::to_string(B) + (which == 0 ? " 1000 bins" : " 7 bins (runs)");
+    CHECK(same, (t + ": repeats bit-identical").c_str());
+    CHECK(exact, (t + ": bits of the documented fold order").c_str());
+  }
+}
+
+void test_cdna4_multi_deterministic()
+{
+  const int N = (1 << 21) + 777;
+  std::vector<double> x(N);
+  for (int i = 0; i < N; ++i) x[i] = (double)(mix(31, i) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  double* dx = to_device(x);
+  check_multi_deterministic<256>(x, dx);
+  check_multi_deterministic<512>(x, dx);
+  // min over FP bins: exact
+  RAJA::MultiReduceMin<RAJA::cdna4_multi_reduce, double> mn(13);
+  RAJA::forall<RAJA::cdna4_exec<256>>(RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+    mn[i % 13].min(dx[i]);
+  });
+  std::vector<double> ref(13, std::numeric_limits<double>::max());
+  for (int i = 0; i < N; ++i) ref[i % 13] = std::min(ref[i % 13], x[i]);
+  bool ok = true;
+  for (int b = 0; b < 13; ++b) ok = ok && mn.get(b) == ref[b];
+  CHECK(ok, "cdna4 FP MultiReduceMin exact");
+  dfree(dx);
+}
+
 void test_cdna4_index_set()
 {
   std::vector<RAJA::Index_type> lh;
@@ -1101,6 +1215,7 @@ int main()
   test_cdna4_launch();
   test_plugins<RAJA::cdna4_exec<256>>("cdna4", RAJA::Platform::cdna4);
   test_cdna4_multi();
+  test_cdna4_multi_deterministic();
   test_cdna4_index_set();
 #endif
   RAJA::util::finalize_plugins();

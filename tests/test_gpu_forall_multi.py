@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "raja_amd", "bin", "forall_multi_reducers")
 
 
-@pytest.mark.parametrize("log2n", [4, 13, 20, 26])
+@pytest.mark.parametrize("log2n", [4, 13, 20, 26, 28])
 def test_forall_multi_reducers(log2n):
     if not os.path.exists(EXE):
         pytest.fail("raja_amd/bin/forall_multi_reducers is not built")

@@ -140,7 +140,9 @@ def test_minloc_ties_and_init(raja, orc):
 
 
 def test_multi_reduce_f64(raja, orc):
-    for n in (1, 777, (1 << 21) + 5):
+    # up to the 2^28 FP64 N(0,1) forall of the scan + multi-reduction workload
+    for n in (1, 777, (1 << 21) + 5, 1 << 28):
+        print("multi_reduce_f64 n=%d" % n, flush=True)
         x = torch.empty(n, dtype=torch.float64, device=DEV)
         raja.fill(raja.FILL_F64_NORMAL, x, seed=5)
         h = to_np(x)

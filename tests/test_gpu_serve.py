@@ -18,7 +18,7 @@ EPS = 2.0 ** -53
 class Pools:
     """Inputs and outputs of a request mix, each request on its own slices."""
 
-    def __init__(self, raja, kinds, sizes, seed):
+    def __init__(self, raja, kinds, sizes, seed, key_bits=None):
         self.kinds, self.sizes = kinds, sizes
         nsum = sum(int(n) for k, n in zip(kinds, sizes) if k == raja.REQ_SUM_F64)
         nscan = sum(int(n) for k, n in zip(kinds, sizes) if k == raja.REQ_EXCLUSIVE_SCAN_I32)
@@ -31,8 +31,12 @@ class Pools:
         raja.fill(raja.FILL_I32, self.scan_in, seed=seed + 1, a=0, b=16)
         self.scan_out = torch.full((nscan + 1,), -7, dtype=torch.int32, device=DEV)
         keys = torch.empty(nsort + 1, dtype=torch.int32, device=DEV)
-        # narrow key range: many duplicates, so stability is exercised
-        raja.fill(raja.FILL_I32, keys, seed=seed + 2, a=0, b=4096)
+        if key_bits is None:
+            # narrow key range: many duplicates, so stability is exercised
+            raja.fill(raja.FILL_I32, keys, seed=seed + 2, a=0, b=4096)
+        else:
+            # uint32 keys uniform over [0, 2^key_bits), stored in int32 tensors
+            raja.fill(raja.FILL_U32_BITS, keys, seed=seed + 2, a=key_bits)
         self.keys = keys
         self.vals = torch.arange(nsort + 1, dtype=torch.int32, device=DEV)
         self.okeys = torch.full((nsort + 1,), -1, dtype=torch.int32, device=DEV)
@@ -182,3 +186,75 @@ def test_serve_arrivals_and_small_arena(raja, orc):
     assert wall >= arrivals[-1]
     check_against_oracle(raja, orc, pools, pools.snapshot())
     srv.close()
+
+
+def check_full_mix(raja, orc, pools, snap):
+    """Every sum within the compensated bound, every scan exact, every sort
+    by an exact size-independent check: the values (iota over the pool) are
+    a permutation of the request's own, the keys follow them, are in order,
+    and equal keys keep input order (the unique stable result)."""
+    sums, scan_out, okeys, ovals = snap
+    x = pools.x.cpu().numpy()
+    sin = pools.scan_in.cpu().numpy()
+    keys = pools.keys.cpu().numpy().view(np.uint32)
+    for i, (k, (o, n)) in enumerate(zip(pools.kinds, pools.spans)):
+        if k == raja.REQ_SUM_F64:
+            kn, ab = orc.kn_sum_f64(x[o:o + n])
+            bound = 2 * max(1, math.ceil(math.log2(max(n, 2)))) * EPS * ab
+            assert abs(sums[i] - kn) <= bound, (i, n)
+        elif k == raja.REQ_EXCLUSIVE_SCAN_I32:
+            want = orc.seq_exclusive(sin[o:o + n], orc.PLUS, 0)
+            assert np.array_equal(scan_out[o:o + n], want), (i, n)
+        else:
+            idx = ovals[o:o + n].astype(np.int64) - o
+            assert idx.min() >= 0 and idx.max() < n, (i, n)
+            assert np.array_equal(np.bincount(idx, minlength=n), np.ones(n, np.int64)), (i, n)
+            ok = okeys[o:o + n].view(np.uint32)
+            assert np.array_equal(ok, keys[o:o + n][idx]), (i, n)
+            assert (ok[1:] >= ok[:-1]).all(), (i, n)
+            tie = ok[1:] == ok[:-1]
+            assert (idx[1:][tie] > idx[:-1][tie]).all(), (i, n)
+
+
+def test_serve_full_mix_at_config_sizes(raja, orc):
+    """The serving workload at its stated shape: 40% FP64 sums, 30% int32
+    exclusive scans, 30% (uint32, int32) sort_pairs with keys uniform over
+    [0, 2^32), n = floor(2^(10 + 14u)) up to 2^24 (one request of each kind
+    at exactly 2^24), Poisson arrivals, a 64 GB arena; batched and
+    per-request modes give the same bits."""
+    rng = np.random.default_rng(71)
+    nreq = 160
+    u = rng.random(nreq)
+    kinds = [int(k) for k in np.where(u < 0.4, raja.REQ_SUM_F64,
+                                      np.where(u < 0.7, raja.REQ_EXCLUSIVE_SCAN_I32,
+                                               raja.REQ_SORT_PAIRS_U32_I32))]
+    sizes = np.floor(2.0 ** (10 + 14 * rng.random(nreq))).astype(np.int64)
+    for j, k in enumerate([raja.REQ_SUM_F64, raja.REQ_EXCLUSIVE_SCAN_I32,
+                           raja.REQ_SORT_PAIRS_U32_I32]):
+        kinds[7 + 40 * j] = k
+        sizes[7 + 40 * j] = 1 << 24
+    arrivals = np.cumsum(rng.exponential(50e-6, nreq))
+    print("serve full mix: %d requests, %d elements" % (nreq, int(sizes.sum())), flush=True)
+    pools = Pools(raja, kinds, sizes, seed=81, key_bits=32)
+    srv = raja.Server(raja.SERVE_BATCHED, streams=16, max_inflight=64, max_batch=512,
+                      batch_limit=1 << 16, arena_bytes=64 << 30, honor_arrivals=True)
+    res, wall = srv.run(pools.requests(raja, arrivals))
+    srv.close()
+    assert all(r.status == 0 for r in res)
+    assert max(r.unit_size for r in res) > 1
+    batched = pools.snapshot()
+    raja.device_status()
+    print("serve full mix: checking batched results", flush=True)
+    check_full_mix(raja, orc, pools, batched)
+
+    print("serve full mix: per-request mode", flush=True)
+    pools.reset_outputs()
+    srv = raja.Server(raja.SERVE_PER_REQUEST, streams=1, max_inflight=8, max_batch=1,
+                      batch_limit=0, arena_bytes=64 << 30, honor_arrivals=True)
+    res, _ = srv.run(pools.requests(raja, arrivals))
+    srv.close()
+    assert all(r.status == 0 and r.unit_size == 1 for r in res)
+    single = pools.snapshot()
+    raja.device_status()
+    for a, b in zip(batched, single):
+        assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")

@@ -220,6 +220,115 @@ def test_segmented_vs_numpy():
     assert np.array_equal(sums, np.add.reduceat(xf, off[:-1]) * (lens > 0))
 
 
+def test_seg_kn_sum_vs_fsum():
+    rng = np.random.default_rng(8)
+    lens = rng.integers(0, 2000, 300)
+    lens[:3] = [0, 1, 0]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = rng.standard_normal(int(off[-1])) * np.exp(rng.uniform(-10, 10, int(off[-1])))
+    s, ab = O.seg_kn_sum_f64(x, off)
+    for g in range(lens.size):
+        seg = x[off[g]:off[g + 1]]
+        exact = math.fsum(seg)
+        assert abs(s[g] - exact) <= 2 * np.spacing(abs(exact)) + 1e-300, g
+        assert ab[g] == pytest.approx(np.abs(seg).sum(), rel=1e-12, abs=0.0)
+    assert s[0] == 0.0 and ab[0] == 0.0 and s[1] == x[off[1]]
+
+
+def test_seg_stable_sort_pairs_vs_numpy():
+    rng = np.random.default_rng(9)
+    lens = rng.integers(0, 700, 150)
+    lens[:2] = [0, 1]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    k = rng.integers(0, 40, int(off[-1])).astype(np.uint32)  # many ties
+    k[off[5]:off[6]] = rng.integers(0, 2 ** 32, int(lens[5]), dtype=np.uint64).astype(np.uint32)
+    v = np.arange(k.size, dtype=np.int32)
+    for desc in (False, True):
+        ko, vo = O.seg_stable_sort_pairs_u32(k, v, off, descending=desc)
+        for g in range(lens.size):
+            a, b = off[g], off[g + 1]
+            seg = k[a:b]
+            # ~key reverses uint32 order and keeps ties in input order
+            order = np.argsort(~seg if desc else seg, kind="stable")
+            assert np.array_equal(vo[a:b], v[a:b][order]), (desc, g)
+            assert np.array_equal(ko[a:b], seg[order]), (desc, g)
+
+
+def test_planted_reversed_descending_sort_is_caught():
+    k = np.array([3, 1, 3, 2, 1], np.uint32)
+    v = np.arange(5, dtype=np.int32)
+    _, vo = O.seg_stable_sort_pairs_u32(k, v, np.array([0, 5], np.int64), descending=True)
+    wrong = np.argsort(k, kind="stable")[::-1]  # ties come out in reverse input order
+    assert not np.array_equal(wrong.astype(np.int32), vo)
+    assert list(vo) == [0, 2, 3, 1, 4]
+
+
+def test_seq_dot_left_fold():
+    # integer-valued data: every product and partial sum is exact
+    rng = np.random.default_rng(10)
+    x = rng.integers(-1000, 1000, 20000).astype(np.float64)
+    y = rng.integers(-1000, 1000, 20000).astype(np.float64)
+    exact = sum(int(a) * int(b) for a, b in zip(x.astype(np.int64), y.astype(np.int64)))
+    assert O.seq_dot_f64(x, y, 0.0) == float(exact)
+    # closed form: sum of i^2 over [0, n)
+    n = 4096
+    i = np.arange(n, dtype=np.float64)
+    assert O.seq_dot_f64(i, i, 5.0) == 5.0 + (n - 1) * n * (2 * n - 1) // 6
+    # seq rounding order: numpy's accumulate is a sequential left fold
+    u = rng.uniform(-1, 1, 30000)
+    w = rng.uniform(-1, 1, 30000)
+    assert O.seq_dot_f64(u, w, 0.0) == np.add.accumulate(u * w)[-1]
+    # a child equal to the identity leaves init untouched
+    z = np.zeros(10)
+    assert O.seq_dot_f64(z, z, 2.5) == 2.5
+
+
+def test_planted_pairwise_dot_is_caught():
+    rng = np.random.default_rng(11)
+    u = rng.uniform(-1, 1, 30000)
+    w = rng.uniform(-1, 1, 30000)
+    pairwise = float(np.sum(u * w))  # numpy's pairwise summation, not a left fold
+    assert pairwise != O.seq_dot_f64(u, w, 0.0)
+
+
+def test_omp_multi_reduce_vs_numpy():
+    rng = np.random.default_rng(12)
+    for nt in (1, 3, 8):
+        x = rng.integers(-50, 50, 100003).astype(np.float64)  # ties across threads
+        s, mn, mx, lv, ll = O.omp_multi_reduce_f64(x, nthreads=nt)
+        assert s == float(int(x.astype(np.int64).sum()))
+        assert mn == x.min() and mx == x.max()
+        assert (lv, ll) == (x.min(), int(np.argmin(x)))
+    z = O.fill_f64_normal(1 << 18, 5)
+    s = O.omp_multi_reduce_f64(z, nthreads=8)[0]
+    exact = math.fsum(z)
+    assert abs(s - exact) <= 2 * 18 * 2.0 ** -53 * np.abs(z).sum()
+    assert O.omp_sum_f64(z, nthreads=8) == pytest.approx(exact, abs=2 * 18 * 2.0 ** -53 * np.abs(z).sum())
+    assert O.omp_sum_f64(np.arange(1000, dtype=np.float64), nthreads=5) == 499500.0
+
+
+def test_omp_daxpy_dot_minloc_vs_numpy():
+    rng = np.random.default_rng(13)
+    x = O.fill_f64_uniform(1 << 16, 0, -1.0, 1.0)
+    y = O.fill_f64_uniform(1 << 16, 1, -1.0, 1.0)
+    y_np = 2.5 * x + y  # numpy: a multiply then an add, two roundings
+    y2 = y.copy()
+    d, lv, ll = O.omp_daxpy_dot_minloc(x, y2, 2.5, nthreads=8)
+    assert np.array_equal(y2, y_np)
+    exact = math.fsum(x * y_np)
+    assert abs(d - exact) <= 2 * 16 * 2.0 ** -53 * np.abs(x * y_np).sum()
+    assert (lv, ll) == (x.min(), int(np.argmin(x)))
+    # integer-valued data: the dot is exact; tied minima resolve to the first index
+    xi = rng.integers(-20, 20, 50001).astype(np.float64)
+    yi = rng.integers(-20, 20, 50001).astype(np.float64)
+    yo = yi.copy()
+    d, lv, ll = O.omp_daxpy_dot_minloc(xi, yo, 2.0, nthreads=8)
+    ye = (2 * xi.astype(np.int64) + yi.astype(np.int64))
+    assert np.array_equal(yo, ye.astype(np.float64))
+    assert d == float(int((xi.astype(np.int64) * ye).sum()))
+    assert ll == int(np.argmin(xi))
+
+
 # ---- halo exchange ------------------------------------------------------------------
 
 def test_halo_lists_golden(golden):
@@ -249,6 +358,11 @@ def test_halo_round_trip():
         untouched = np.ones(size, bool)
         untouched[ulist] = False
         assert np.array_equal(vars_[v][untouched], orig[v][untouched])
+    # the omp variant moves the same bytes
+    vars2 = [o.copy() for o in orig]
+    O.omp_halo_cycle(vars2, plist, ulist, plen, nthreads=4)
+    for v in range(nv):
+        assert np.array_equal(vars2[v], vars_[v])
 
 
 # ---- planted mistakes: each fails a check used above ----------------------------------
