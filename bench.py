@@ -13,7 +13,8 @@ sort_pairs of keys in [0, 2^12), all at N = 2^28.
 
 The CPU baseline is the oracle's omp_parallel_for_exec restatement of RAJA's
 stable sort_pairs on a 2^26-element sample of the same distribution, timed on
-this host's cores, scaled to the same GB/s definition.
+this host's cores, scaled to the same GB/s definition; the reduce, scan,
+forall and halo restatements are timed beside it (cpu_baseline.ops).
 """
 import argparse
 import json
@@ -25,6 +26,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads: one per core, spread (read when the
+# OpenMP runtime loads, so set before torch or the oracle is imported).
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 PEAK_GBS = 8000.0
 METRIC = "GB/s & %HBM peak: reduce/scan/sort_pairs 2^28; batched req/s + p99 latency"
@@ -79,6 +84,9 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
     n = 1 << args.log2n
+
+    # ---- shapes: measured (or read from the on-disk cache) for the timed ops -------
+    shapes = tune_shapes(R, torch, n)
 
     # ---- headline: sort_pairs u32 uniform, iota values ------------------------------
     log("bench: sort_pairs n=2^%d setup" % args.log2n)
@@ -208,6 +216,7 @@ def main():
             "hist_kernel_ms": round(hist["total_ms"] / max(1, hist["count"]), 4),
         },
         "cpu_baseline": cpu,
+        "shapes": shapes,
         "ops": ops,
         "serving": serve,
         "workgroup_halo": halo,
@@ -215,6 +224,27 @@ def main():
         "forall_sum_min_max_minloc": fm,
     }
     print(json.dumps(line), flush=True)
+
+
+def tune_shapes(R, torch, n):
+    """Shapes of the timed scans and the sort at length n: read from the
+    shape cache, measured (raja_cdna4_autotune) where the cache has none."""
+    out = {"cache": R.tune_cache_path()}
+    for name, op, dt in (("sort_pairs_u32", R.TUNE_SORT_PAIRS, torch.uint32),
+                         ("exclusive_scan_i32", R.TUNE_EXCLUSIVE_SCAN, torch.int32),
+                         ("inclusive_scan_i64", R.TUNE_INCLUSIVE_SCAN, torch.int64)):
+        choice, tuned = R.tune_query(op, dt, n)
+        measured = False
+        if not tuned:
+            try:
+                R.autotune(op, dt, n)
+                measured = True
+            except R.RajaError as e:  # e.g. an unwritable cache directory
+                log("bench: shape tuning of %s: %s" % (name, e))
+            choice, tuned = R.tune_query(op, dt, n)
+        out[name] = {"choice": choice, "tuned": tuned, "measured_now": measured}
+        log("bench: shape %s at n=%d: %s (0 = default)" % (name, n, choice))
+    return out
 
 
 TRAFFIC_FILE = os.path.join("profiles", "onesweep_pass_sort_pairs_2p28_hbm_traffic.md")
@@ -487,28 +517,98 @@ def serving(R, torch, np, dev, nreq=4000):
     return out
 
 
+def _host_info(O):
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "omp_max_threads": O.omp_max_threads(),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def _cpu_time(fn, reps=5, prep=None):
+    times = []
+    for _ in range(reps):
+        if prep is not None:
+            prep()
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
 def cpu_baseline(np):
-    """omp_parallel_for_exec stable sort_pairs (oracle restatement), 2^26 sample."""
+    """The oracle's omp_parallel_for_exec restatements (RAJA's OpenMP
+    backend semantics) timed on this host's cores, median of 5, on bounded
+    samples of each workload; GB/s uses the GPU's algorithmic byte counts.
+    The headline entry is the stable sort_pairs."""
     from oracle import oracle as O
-    m = 1 << 26
     threads = O.omp_max_threads()
+    host = _host_info(O)
+    ops = {}
+
+    def rec(name, sec, nbytes, sample):
+        ops[name] = {"s": round(sec, 5), "GB/s": round(nbytes / sec / 1e9, 3), "sample": sample}
+        log("bench: cpu omp %-26s %.4f s  %.2f GB/s  (%s)" % (name, sec, nbytes / sec / 1e9, sample))
+
+    m = 1 << 26
     keys = O.fill_u32_bits(m, 1, 32)
     vals = np.arange(m, dtype=np.int32)
-    times = []
-    for _ in range(3):
-        k = keys.copy()
-        v = vals.copy()
-        t0 = time.perf_counter()
-        O.omp_stable_sort_pairs(k, v, nthreads=threads)
-        times.append(time.perf_counter() - t0)
-    s = statistics.median(times)
+    k, v = keys.copy(), vals.copy()
+
+    def prep():
+        k[:] = keys
+        v[:] = vals
+    s = _cpu_time(lambda: O.omp_stable_sort_pairs(k, v, nthreads=threads), prep=prep)
     gbs = m * 68 / s / 1e9
     log("bench: cpu omp stable_sort_pairs 2^26: %.3f s (%.2f GB/s equiv, %d threads)" % (s, gbs, threads))
+    del keys, vals, k, v
+
+    x = O.fill_f64_normal(m, 5)
+    rec("multi_reduce_f64_sum_min_max_minloc", _cpu_time(lambda: O.omp_multi_reduce_f64(x, threads)),
+        8 * m, "2^26 N(0,1) FP64, seed 5")
+    rec("reduce_sum_f64", _cpu_time(lambda: O.omp_sum_f64(x, threads)), 8 * m, "2^26 N(0,1) FP64")
+    del x
+    xi = O.fill_i32_range(m, 3, 0, 16)
+    oi = np.empty_like(xi)
+    rec("exclusive_scan_i32", _cpu_time(lambda: O.omp_exclusive_i32(xi, oi, 0, threads)), 8 * m,
+        "2^26 int32 in [0,16), seed 3")
+    del xi, oi
+    xl = O.fill_i64_range(m, 4, 0, 16)
+    ol = np.empty_like(xl)
+    rec("inclusive_scan_i64", _cpu_time(lambda: O.omp_inclusive_i64(xl, ol, threads)), 16 * m,
+        "2^26 int64 in [0,16), seed 4")
+    del xl, ol
+
+    nd = 1 << 24
+    xd = O.fill_f64_uniform(nd, 0, -1.0, 1.0)
+    yd = O.fill_f64_uniform(nd, 10, -1.0, 1.0)
+    rec("daxpy_dot_minloc_f64", _cpu_time(lambda: O.omp_daxpy_dot_minloc(xd, yd, 2.5, threads)),
+        (24 + 16) * nd, "2^24 FP64 (the full forall example size): daxpy then dot + MinLoc")
+    del xd, yd
+
+    g, h, nv = 256, 2, 10
+    plen, plists = O.halo_lists([g, g, g], h, False)
+    _, ulists = O.halo_lists([g, g, g], h, True)
+    cells = (g + 2 * h) ** 3
+    vars_ = [np.arange(cells, dtype=np.float64) + vi for vi in range(nv)]
+    buf = np.empty(int(plen.sum()) * nv, np.float64)
+    rec("halo_pack_unpack_cycle", _cpu_time(lambda: O.omp_halo_cycle(vars_, plists, ulists, plen, buf,
+                                                                      threads)),
+        2 * 20 * int(plen.sum()) * nv, "256^3 grid, halo 2, 10 FP64 variables (the full size)")
+    del vars_, buf
+
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": "stable sort_pairs of 2^26 uint32 keys uniform [0,2^32) + int32 iota "
-                      "(same generator, seed 1); median of 3; GB/s = 68 B/elem x N / time, "
+                      "(same generator, seed 1); median of 5; GB/s = 68 B/elem x N / time, "
                       "the GPU's algorithmic byte count",
-            "seconds": round(s, 3), "keys_per_s": round(m / s, 1)}
+            "seconds": round(s, 3), "keys_per_s": round(m / s, 1), "host": host, "ops": ops}
 
 
 if __name__ == "__main__":

@@ -189,6 +189,45 @@ int raja_cdna4_exclusive_scan(int dtype, int op, const void* in, void* out,
 int raja_cdna4_sort_pairs(int key_dtype, void* keys, void* vals, int val_bytes,
                           int64_t n, int descending, void* stream);
 
+/* ---- shape tuning ------------------------------------------------------------- */
+/* The reference chooses launch shapes at run time with its occupancy
+ * calculator and concretizers (include/RAJA/policy/hip/MemUtils_HIP.hpp:
+ * 389-519, 547-633) and at compile time with tuning policies
+ * (include/RAJA/policy/hip/policy.hpp:1228-1353).  Here the tile of a span
+ * scan (vectors per lane) and of a radix pass (keys per thread, 4-byte keys)
+ * are measured per (operation, element type, length bucket ceil(log2 n)) on
+ * the current device and kept in a text file named for the device's
+ * architecture, CU count and HIP runtime version, in $RAJA_CDNA4_TUNE_DIR
+ * (else $XDG_CACHE_HOME/raja_cdna4, else $HOME/.cache/raja_cdna4).  Scans and
+ * sorts look their bucket up on every call and take the default shape when
+ * it is absent; with RAJA_CDNA4_AUTOTUNE=1 in the environment an absent
+ * bucket is measured at its first call instead (never under graph capture).
+ * Integer scans and sorts give the same bits in every shape, so the file is
+ * safe to delete: only speed depends on it. */
+enum raja_cdna4_tune_op
+{
+  RAJA_CDNA4_TUNE_INCLUSIVE_SCAN = 0,
+  RAJA_CDNA4_TUNE_EXCLUSIVE_SCAN = 1,
+  RAJA_CDNA4_TUNE_SORT_PAIRS = 2
+};
+
+/* Measure every candidate shape of (tune_op, dtype) on n synthetic elements
+ * (allocated for the measurement) on `stream`, record the fastest for n's
+ * bucket and rewrite the cache file.  Synchronous. */
+int raja_cdna4_autotune(int tune_op, int dtype, int64_t n, void* stream);
+
+/* The shape a call of (tune_op, dtype, n) takes: *choice 0 is the default
+ * shape, else vectors per lane (scans) or keys per thread (sorts); *tuned is
+ * 1 when the bucket has a recorded measurement. */
+int raja_cdna4_tune_query(int tune_op, int dtype, int64_t n, int* choice, int* tuned);
+
+/* Path of the current device's cache file ("" when no directory is known). */
+int raja_cdna4_tune_cache_path(char* buf, size_t len);
+
+/* Forget every recorded shape in this process; the file is read again at
+ * the next call. */
+int raja_cdna4_tune_forget(void);
+
 /* ---- batched (segmented) work ---------------------------------------------- */
 /* No single reference entry point: these replace one reference algorithm call
  * per range -- a forall with ReduceSum, an exclusive_scan, a sort_pairs

@@ -96,6 +96,10 @@ def _declare(L):
         "raja_cdna4_inclusive_scan": [i32, i32, vp, vp, i64, vp],
         "raja_cdna4_exclusive_scan": [i32, i32, vp, vp, i64, vp, vp],
         "raja_cdna4_sort_pairs": [i32, vp, vp, i32, i64, i32, vp],
+        "raja_cdna4_autotune": [i32, i32, i64, vp],
+        "raja_cdna4_tune_query": [i32, i32, i64, ctypes.POINTER(i32), ctypes.POINTER(i32)],
+        "raja_cdna4_tune_cache_path": [ctypes.c_char_p, ctypes.c_size_t],
+        "raja_cdna4_tune_forget": [],
         "raja_cdna4_timing_enable": [i32],
         "raja_cdna4_timing_reset": [],
         "raja_cdna4_timing_query": [ctypes.c_char_p, ctypes.POINTER(i64),
@@ -494,6 +498,39 @@ def segmented_sort_pairs(plan, keys, vals=None, comparator=operators.less):
         plan.handle, _dt(keys), keys.data_ptr(), None if vals is None else vals.data_ptr(), vb,
         1 if comparator == operators.greater else 0, s))
     return keys, vals
+
+
+# ---- shape tuning (include/raja_cdna4.h: raja_cdna4_autotune) ----------------------
+
+TUNE_INCLUSIVE_SCAN, TUNE_EXCLUSIVE_SCAN, TUNE_SORT_PAIRS = 0, 1, 2
+
+
+def autotune(op, dtype, n):
+    """Measure the candidate shapes of (op, dtype) at length n on the current
+    device's current stream and record the fastest in the on-disk cache
+    (synchronous)."""
+    s = _prep()
+    _check(lib().raja_cdna4_autotune(op, _DTYPES[dtype], int(n), s))
+
+
+def tune_query(op, dtype, n):
+    """(choice, tuned): the shape a call of (op, dtype, n) takes; choice 0 is
+    the default shape."""
+    _prep()
+    c, t = ctypes.c_int(), ctypes.c_int()
+    _check(lib().raja_cdna4_tune_query(op, _DTYPES[dtype], int(n), ctypes.byref(c), ctypes.byref(t)))
+    return c.value, bool(t.value)
+
+
+def tune_cache_path():
+    _prep()
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib().raja_cdna4_tune_cache_path(buf, len(buf)))
+    return buf.value.decode()
+
+
+def tune_forget():
+    _check(lib().raja_cdna4_tune_forget())
 
 
 # ---- synthetic inputs -------------------------------------------------------------

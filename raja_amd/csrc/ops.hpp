@@ -13,6 +13,8 @@
 #include <float.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace rk
 {
 
@@ -221,5 +223,17 @@ struct OpMulti
     a.ll = take ? i : a.ll;
   }
 };
+
+// enum raja_cdna4_dtype code of an element type.
+template<typename T>
+constexpr int dtype_code()
+{
+  if constexpr (std::is_same_v<T, int32_t>) return 0;
+  else if constexpr (std::is_same_v<T, int64_t>) return 1;
+  else if constexpr (std::is_same_v<T, uint32_t>) return 2;
+  else if constexpr (std::is_same_v<T, uint64_t>) return 3;
+  else if constexpr (std::is_same_v<T, float>) return 4;
+  else return 5;
+}
 
 }  // namespace rk

@@ -24,6 +24,7 @@
 #include "ops.hpp"
 #include "runtime.hpp"
 #include "segments.hpp"
+#include "tune.hpp"
 #include "workspace.hpp"
 
 namespace
@@ -39,7 +40,8 @@ constexpr int64_t kShortMeanLen = 65536;
 
 // Tile shapes: MODE 0 a span, MODE 1 a tile table (batched requests, long
 // segments), MODE 2 a tile table of mostly short segments.
-template<typename T, int MODE = 0>
+// VO != 0 overrides the vectors per lane of a span (a tuned shape).
+template<typename T, int MODE = 0, int VO = 0>
 struct ScanShape
 {
   static constexpr int VEC = 16 / sizeof(T);
@@ -50,7 +52,8 @@ struct ScanShape
   // their segments are short on average: most of their tiles are partial
   // (20000 log-uniform segments of mean length 5800, int32: 0.160 ms at 4,
   // 0.170 at 6, 0.185 at 8, 0.200 at 12, 0.22 at 16).
-  static constexpr int V = sizeof(T) == 8 ? 12 : (MODE == 0 ? 16 : (MODE == 1 ? 8 : 4));
+  static constexpr int V =
+      VO != 0 ? VO : (sizeof(T) == 8 ? 12 : (MODE == 0 ? 16 : (MODE == 1 ? 8 : 4)));
   static constexpr int WCHUNK = wave_size * V * VEC;   // elements per wave
   static constexpr int64_t TILE = (int64_t)kWaves * WCHUNK;
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
@@ -203,14 +206,14 @@ __device__ __forceinline__ T lookback(const Desc<T>* desc, int64_t tile, int64_t
 // MODE != 0 (SEG): the tiles of many ranges (segments) come from a tile
 // table; each range starts from `init` and its lookback stops at its own
 // first tile.
-template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED, int MODE>
+template<typename T, typename Op, bool EXCLUSIVE, bool ALIGNED, int MODE, int VO = 0>
 __global__ __launch_bounds__(kBlock) void scan_kernel(const T* in, T* out, int64_t n, T init,
                                                       Desc<T>* __restrict__ desc,
                                                       uint32_t epoch_tag,
                                                       const rs::TileRec* __restrict__ tiles)
 {
   constexpr bool SEG = MODE != 0;
-  using S = ScanShape<T, MODE>;
+  using S = ScanShape<T, MODE, VO>;
   using vec_t = typename S::vec_t;
   constexpr int VEC = S::VEC;
   constexpr int V = S::V;
@@ -441,10 +444,23 @@ int table_mode(const raja_cdna4_segments_s* plan)
 // tiles); otherwise tile table records of shape `mode` (1 or 2).  desc holds
 // ntiles descriptors; with `clear` they are zeroed first (required whenever
 // the memory may hold this epoch tag from an earlier use).
+// Span tile of 4- and 8-byte values for a tuned vectors-per-lane choice
+// (0: the default shape).  The tuned shapes are taken by aligned spans.
+template<typename T>
+constexpr int64_t span_tile(int v)
+{
+  return (int64_t)kWaves * wave_size * (v != 0 ? v : ScanShape<T>::V) * ScanShape<T>::VEC;
+}
+
+// The vectors-per-lane choices the tuner measures for a span of T (the
+// default first); any other value is taken as the default.
+template<typename T>
+constexpr int kSpanChoices[3] = {0, sizeof(T) == 8 ? 6 : 8, sizeof(T) == 8 ? 3 : 4};
+
 template<typename T, typename Op, bool EXCLUSIVE>
 int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
                      const rs::TileRec* tiles, int64_t ntiles, void* desc, uint32_t epoch_tag,
-                     bool clear, int mode = 1)
+                     bool clear, int mode = 1, int v = 0)
 {
   const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
   if (clear)
@@ -471,6 +487,18 @@ int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, cons
     hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, false, 1>), dim3((unsigned)ntiles),
                        dim3(kBlock), 0, s, in, out, n, init, (Desc<T>*)desc, epoch_tag, tiles);
   }
+  else if (aligned && v == kSpanChoices<T>[1])
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, 0, kSpanChoices<T>[1]>),
+                       dim3((unsigned)ntiles), dim3(kBlock), 0, s, in, out, n, init,
+                       (Desc<T>*)desc, epoch_tag, (const rs::TileRec*)nullptr);
+  }
+  else if (aligned && v == kSpanChoices<T>[2])
+  {
+    hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, 0, kSpanChoices<T>[2]>),
+                       dim3((unsigned)ntiles), dim3(kBlock), 0, s, in, out, n, init,
+                       (Desc<T>*)desc, epoch_tag, (const rs::TileRec*)nullptr);
+  }
   else if (aligned)
   {
     hipLaunchKernelGGL((scan_kernel<T, Op, EXCLUSIVE, true, 0>), dim3((unsigned)ntiles),
@@ -490,15 +518,27 @@ int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, cons
 // table == nullptr: one span of n elements; otherwise the tiles of a
 // segment plan (n = total elements of the plan).  Descriptors come from the
 // stream's scratch.
+// A span takes the tuned shape of its (kind, dtype, length bucket) when both
+// pointers are 16-byte aligned (v < 0: look it up; v >= 0: this shape).
 template<typename T, typename Op, bool EXCLUSIVE>
 int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
-                const rs::TileTable* table = nullptr, int mode = 1)
+                const rs::TileTable* table = nullptr, int mode = 1, int v = -1)
 {
   if (n < 0) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "negative length");
   if (n == 0) return RAJA_CDNA4_SUCCESS;  // empty range: no-op (pattern/scan.hpp:72-75)
   if (!in || !out) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
-  using S = ScanShape<T>;
-  const int64_t ntiles = table ? table->ntiles : rc::ceil_div(n, S::TILE);
+  int vv = 0;
+  if (!table && (((uintptr_t)in | (uintptr_t)out) % 16) == 0)
+  {
+    vv = v;
+    if (vv < 0)
+    {
+      RC_TRY(rtune::choice(EXCLUSIVE ? rtune::kExclusiveScan : rtune::kInclusiveScan,
+                           rk::dtype_code<T>(), n, s, &vv));
+    }
+    if (vv != kSpanChoices<T>[1] && vv != kSpanChoices<T>[2]) vv = 0;
+  }
+  const int64_t ntiles = table ? table->ntiles : rc::ceil_div(n, span_tile<T>(vv));
   if (ntiles == 0) return RAJA_CDNA4_SUCCESS;
   if (ntiles > 0x7FFFFFFF) return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan too long");
   rc::StreamCtx* c = nullptr;
@@ -514,7 +554,7 @@ int launch_scan(const T* in, T* out, int64_t n, T init, hipStream_t s, const cha
   RC_HIP(hipStreamIsCapturing(s, &cap));
   return launch_scan_core<T, Op, EXCLUSIVE>(
       in, out, n, init, s, name, table ? (const rs::TileRec*)table->d_tiles : nullptr, ntiles,
-      c->scan_desc, epoch_tag, cap != hipStreamCaptureStatusNone, mode);
+      c->scan_desc, epoch_tag, cap != hipStreamCaptureStatusNone, mode, vv);
 }
 
 template<typename T, bool EXCLUSIVE>
@@ -639,6 +679,35 @@ int seg_scan_dispatch(int dtype, int op, const void* in, void* out, raja_cdna4_s
 
 namespace rscan
 {
+void span_choices(int dtype, int out[3])
+{
+  const bool wide = dtype == RAJA_CDNA4_I64 || dtype == RAJA_CDNA4_U64 || dtype == RAJA_CDNA4_F64;
+  for (int i = 0; i < 3; ++i) out[i] = wide ? kSpanChoices<int64_t>[i] : kSpanChoices<int32_t>[i];
+}
+
+int tune_scan(int kind_exclusive, int dtype, const void* in, void* out, int64_t n, int v,
+              hipStream_t s)
+{
+  switch (dtype)
+  {
+#define RSCAN_TUNE_CASE(CODE, T)                                                               \
+  case CODE:                                                                                   \
+    return kind_exclusive                                                                      \
+               ? launch_scan<T, rk::OpPlus<T>, true>((const T*)in, (T*)out, n, T(0), s,         \
+                                                     "exclusive_scan", nullptr, 1, v)          \
+               : launch_scan<T, rk::OpPlus<T>, false>((const T*)in, (T*)out, n, T(0), s,        \
+                                                      "inclusive_scan", nullptr, 1, v);
+    RSCAN_TUNE_CASE(RAJA_CDNA4_I32, int32_t)
+    RSCAN_TUNE_CASE(RAJA_CDNA4_I64, int64_t)
+    RSCAN_TUNE_CASE(RAJA_CDNA4_U32, uint32_t)
+    RSCAN_TUNE_CASE(RAJA_CDNA4_U64, uint64_t)
+    RSCAN_TUNE_CASE(RAJA_CDNA4_F32, float)
+    RSCAN_TUNE_CASE(RAJA_CDNA4_F64, double)
+#undef RSCAN_TUNE_CASE
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "dtype %d not supported", dtype);
+  }
+}
+
 int take_spin_timeouts(uint32_t* count)
 {
   uint32_t v = 0;

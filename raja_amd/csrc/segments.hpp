@@ -34,6 +34,19 @@ struct SortSeg
   int64_t n;
 };
 
+// One tile of one range sorted by the segmented onesweep: element offsets of
+// the range's keys and values in the input (ik, iv), output (ok, ov) and
+// alternate (ak, av) buffers, each from that buffer's base pointer; the
+// range length, the tile's index inside the range and the range's index
+// (its digit-histogram row).
+struct SortTile
+{
+  int64_t ik, iv, ok, ov, ak, av;
+  int64_t len;
+  uint32_t j;
+  uint32_t seg;
+};
+
 // Tile records of segments given as (input start, length, output start).
 inline void append_tiles(std::vector<TileRec>& recs, int64_t start, int64_t len, int64_t out_start,
                          uint32_t seg, int64_t tile)

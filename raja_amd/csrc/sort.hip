@@ -25,6 +25,8 @@
 #include "ops.hpp"
 #include "radix.hpp"
 #include "runtime.hpp"
+#include "segments.hpp"
+#include "tune.hpp"
 #include "workspace.hpp"
 
 namespace
@@ -193,11 +195,21 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
 // scans, look-back) is paid once per 16384 keys: larger tiles measured
 // faster than more resident blocks.  The keys are staged and the values
 // loaded before the look-back, so both overlap its latency.
-template<typename K, typename V, bool DESC, bool HAS_VALS, int BLOCK, int ITEMS>
+// SEG: the tiles of many ranges (one table record per tile, see
+// rs::SortTile) sorted in one launch per pass.  keys_in / vals_in /
+// keys_out / vals_out are then the bases of the input, output and
+// alternate buffers in the order of `bases` below; a range reads the pass's
+// source and writes its destination (pass 0: input -> alternate, odd
+// passes: alternate -> output, even ones: output -> alternate), takes its
+// digit offsets from its own histogram row and looks back only as far as
+// its own first tile.
+template<typename K, typename V, bool DESC, bool HAS_VALS, int BLOCK, int ITEMS, bool SEG = false>
 __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const K* __restrict__ keys_in, K* __restrict__ keys_out, const V* __restrict__ vals_in,
     V* __restrict__ vals_out, int64_t n, int shift, const uint32_t* __restrict__ hist,
-    uint32_t* __restrict__ desc_cur, uint32_t* __restrict__ desc_next)
+    uint32_t* __restrict__ desc_cur, uint32_t* __restrict__ desc_next,
+    const rs::SortTile* __restrict__ recs = nullptr, int pass = 0,
+    K* __restrict__ keys_alt = nullptr, V* __restrict__ vals_alt = nullptr)
 {
   static_assert(BLOCK >= kRadix && BLOCK % wave_size == 0, "one thread per digit");
   constexpr int NW = BLOCK / wave_size;
@@ -233,7 +245,28 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const int64_t r = b - g * GX;
     return g * GX + (r & 7) * KX + (r >> 3);
   }();
-  const int64_t tbase = tile * TILE;
+  int64_t tj = tile;     // tile index inside its range
+  int64_t tfirst = 0;    // descriptor row of the range's first tile
+  if constexpr (SEG)
+  {
+    const rs::SortTile r = recs[tile];
+    tj = r.j;
+    tfirst = tile - r.j;
+    n = r.len;
+    hist += (int64_t)r.seg * (int64_t)sizeof(K) * kRadix;
+    const K* ks = pass == 0 ? keys_in + r.ik : ((pass & 1) ? keys_alt + r.ak : keys_out + r.ok);
+    K* kd = (pass & 1) ? keys_out + r.ok : keys_alt + r.ak;
+    keys_in = ks;
+    keys_out = kd;
+    if (HAS_VALS)
+    {
+      const V* vs = pass == 0 ? vals_in + r.iv : ((pass & 1) ? vals_alt + r.av : vals_out + r.ov);
+      V* vd = (pass & 1) ? vals_out + r.ov : vals_alt + r.av;
+      vals_in = vs;
+      vals_out = vd;
+    }
+  }
+  const int64_t tbase = tj * TILE;
   const int64_t wbase = tbase + (int64_t)wave * WCHUNK;
 
   // Item j of this lane is element wbase + j*64 + lane; the wave holds
@@ -310,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       s_whist[w * kRadix + d] = total;
       total += c;
     }
-    atomic_store_agent(&row[d], ((tile == 0 ? kInc : kAgg) << 30) | total);
+    atomic_store_agent(&row[d], ((tj == 0 ? kInc : kAgg) << 30) | total);
     const uint32_t gcount = hist[d];
     uint32_t tot_t, tot_g;
     (void)wave_scan(total, rk::OpPlus<uint32_t>{}, 0u, ex_t, tot_t);
@@ -368,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
       }
     }
   }
-  if (tile > 0)
+  if (tj > 0)
   {
     if (tid < kRadix)
     {
@@ -385,14 +418,14 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
         for (int u = 0; u < 8; ++u)
         {
           const int64_t t = t0 - u;
-          w[u] = (t >= 0) ? atomic_load_agent(&desc_cur[t * kRowStride + d]) : (kInc << 30);
+          w[u] = (t >= tfirst) ? atomic_load_agent(&desc_cur[t * kRowStride + d]) : (kInc << 30);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
         {
           if (done) break;
           const int64_t t = t0 - u;
-          if (t < 0)
+          if (t < tfirst)
           {
             done = true;
             break;
@@ -511,6 +544,11 @@ struct SortShape
   static constexpr int BLOCK = 512;
   static constexpr int ITEMS = wide ? 16 : 32;  // 8192 / 16384 keys per tile
   static constexpr int64_t TILE = (int64_t)BLOCK * ITEMS;
+  // Keys per thread the tuner may choose instead (4-byte keys and values
+  // only; 0 is the default): smaller tiles give mid-sized sorts enough
+  // blocks to fill the chip.
+  static constexpr int kChoices[3] = {0, wide ? 0 : 16, wide ? 0 : 8};
+  static int64_t tile(int items) { return (int64_t)BLOCK * (items > 0 ? items : ITEMS); }
 };
 
 // Working memory of one sort: alternate key/value buffers, the digit
@@ -522,9 +560,9 @@ struct SortWs
   static constexpr int P = (int)sizeof(K);
   int64_t n, ntiles, desc_words;
   size_t keys_bytes, vals_bytes, hist_bytes, desc_bytes;
-  explicit SortWs(int64_t n_) : n(n_)
+  explicit SortWs(int64_t n_, int items = 0) : n(n_)
   {
-    ntiles = rc::ceil_div(n, S::TILE);
+    ntiles = rc::ceil_div(n, S::tile(items));
     desc_words = ntiles * kRowStride;
     keys_bytes = rc::align_up((size_t)n * sizeof(K), 256);
     vals_bytes = HAS_VALS ? rc::align_up((size_t)n * sizeof(V), 256) : 0;
@@ -535,6 +573,76 @@ struct SortWs
   size_t total() const { return tmp_bytes() + desc_bytes; }
 };
 
+// ---- segmented onesweep: the tiles of many ranges, one launch per pass ----------
+// Digit histograms of every range of a tile table: one block per tile adds
+// its keys' digit counts to its range's P rows of 256 counters, and clears
+// the tile's descriptor row of the first pass.
+template<typename K, bool DESC, int P>
+__global__ __launch_bounds__(256) void seg_hist_kernel(const K* __restrict__ keys,
+                                                       const rs::SortTile* __restrict__ recs,
+                                                       int64_t tile_elems,
+                                                       uint32_t* __restrict__ hist,
+                                                       uint32_t* __restrict__ desc0)
+{
+  __shared__ uint32_t s_hist[P * kRadix];
+  for (int i = threadIdx.x; i < P * kRadix; i += 256) s_hist[i] = 0u;
+  const rs::SortTile r = recs[blockIdx.x];
+  __syncthreads();
+  const int64_t first = (int64_t)r.j * tile_elems;
+  const int64_t rem = r.len - first;
+  const int cnt = (int)(rem < tile_elems ? rem : tile_elems);
+  const K* kp = keys + r.ik + first;
+  for (int i = threadIdx.x; i < cnt; i += 256)
+  {
+    const K k = kp[i];
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&s_hist[p * kRadix + digit_of<K, DESC>(k, 8 * p)], 1u);
+  }
+  __syncthreads();
+  uint32_t* const h = hist + (int64_t)r.seg * P * kRadix;
+  for (int i = threadIdx.x; i < P * kRadix; i += 256)
+  {
+    const uint32_t c = s_hist[i];
+    if (c) atomic_add_agent(&h[i], c);
+  }
+  for (int i = threadIdx.x; i < kRowStride; i += 256) desc0[(int64_t)blockIdx.x * kRowStride + i] = 0u;
+}
+
+// Sorts every range of a tile table (records built with tile SortShape::TILE)
+// from the input into the output buffers, through the alternates; P is even,
+// so the last pass lands in the output.  hist holds nranges * P * 256
+// counters, desc 2 * ntiles descriptor rows; both need no clearing.
+template<typename K, typename V, bool HAS_VALS, bool DESC>
+int seg_sort_core(const K* in_k, const V* in_v, K* out_k, V* out_v, K* alt_k, V* alt_v,
+                  const rs::SortTile* d_recs, int64_t ntiles, int64_t nranges, uint32_t* hist,
+                  uint32_t* desc, hipStream_t s)
+{
+  using S = SortShape<K, V, HAS_VALS>;
+  constexpr int P = (int)sizeof(K);
+  static_assert(P % 2 == 0, "an even pass count ends in the output");
+  if (ntiles <= 0) return RAJA_CDNA4_SUCCESS;
+  if (ntiles > 0x7FFFFFFF) return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "too many tiles");
+  rc::TimedScope ts("segmented_onesweep", s);
+  const int64_t words = nranges * P * kRadix;
+  const int64_t cb = rc::ceil_div(words, 256);
+  hipLaunchKernelGGL(clear_words_kernel, dim3((unsigned)(cb < 1024 ? cb : 1024)), dim3(256), 0, s,
+                     hist, words);
+  hipLaunchKernelGGL((seg_hist_kernel<K, DESC, P>), dim3((unsigned)ntiles), dim3(256), 0, s, in_k,
+                     d_recs, S::TILE, hist, desc);
+  RC_HIP(hipGetLastError());
+  uint32_t* const d0 = desc;
+  uint32_t* const d1 = desc + ntiles * kRowStride;
+  for (int p = 0; p < P; ++p)
+  {
+    hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::ITEMS, true>),
+                       dim3((unsigned)ntiles), dim3(S::BLOCK), 0, s, in_k, out_k, in_v, out_v,
+                       (int64_t)0, 8 * p, hist + p * kRadix, (p & 1) ? d1 : d0, (p & 1) ? d0 : d1,
+                       d_recs, p, alt_k, alt_v);
+    RC_HIP(hipGetLastError());
+  }
+  return RAJA_CDNA4_SUCCESS;
+}
+
 // Sort (in_keys, in_vals) into (keys, vals); in_keys == keys sorts in place.
 // tmp holds SortWs::tmp_bytes(), desc_mem SortWs::desc_bytes (its contents
 // are ignored: the histogram kernel clears the first array and every pass
@@ -543,11 +651,12 @@ struct SortWs
 // whose digit is the same for every key.
 template<typename K, typename V, bool HAS_VALS, bool DESC>
 int sort_core(const K* in_keys, const V* in_vals, K* keys, V* vals, int64_t n, hipStream_t s,
-              char* tmp, uint32_t* desc_mem, uint32_t* pinned)
+              char* tmp, uint32_t* desc_mem, uint32_t* pinned, int items = 0)
 {
   using S = SortShape<K, V, HAS_VALS>;
   constexpr int P = (int)sizeof(K);
-  const SortWs<K, V, HAS_VALS> w(n);
+  if (items != S::kChoices[1] && items != S::kChoices[2]) items = 0;
+  const SortWs<K, V, HAS_VALS> w(n, items);
   K* alt_keys = (K*)tmp;
   V* alt_vals = HAS_VALS ? (V*)(tmp + w.keys_bytes) : nullptr;
   uint32_t* hist = (uint32_t*)(tmp + w.keys_bytes + w.vals_bytes);
@@ -598,9 +707,24 @@ int sort_core(const K* in_keys, const V* in_vals, K* keys, V* vals, int64_t n, h
     K* kdst = to_alt ? alt_keys : keys;
     V* vdst = to_alt ? alt_vals : vals;
     rc::TimedScope ts("onesweep_pass", s);
-    hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::ITEMS>),
-                       dim3((unsigned)w.ntiles), dim3(S::BLOCK), 0, s, (K*)ksrc, kdst, (V*)vsrc, vdst,
-                       n, 8 * p, hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
+    if (items != 0 && items == S::kChoices[1])
+    {
+      hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::kChoices[1] ? S::kChoices[1] : S::ITEMS>),
+                         dim3((unsigned)w.ntiles), dim3(S::BLOCK), 0, s, (K*)ksrc, kdst, (V*)vsrc,
+                         vdst, n, 8 * p, hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
+    }
+    else if (items != 0 && items == S::kChoices[2])
+    {
+      hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::kChoices[2] ? S::kChoices[2] : S::ITEMS>),
+                         dim3((unsigned)w.ntiles), dim3(S::BLOCK), 0, s, (K*)ksrc, kdst, (V*)vsrc,
+                         vdst, n, 8 * p, hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
+    }
+    else
+    {
+      hipLaunchKernelGGL((onesweep_kernel<K, V, DESC, HAS_VALS, S::BLOCK, S::ITEMS>),
+                         dim3((unsigned)w.ntiles), dim3(S::BLOCK), 0, s, (K*)ksrc, kdst, (V*)vsrc,
+                         vdst, n, 8 * p, hist + p * kRadix, desc[q & 1], desc[(q + 1) & 1]);
+    }
     RC_HIP(hipGetLastError());
     ksrc = kdst;
     vsrc = vdst;
@@ -618,18 +742,26 @@ int sort_core(const K* in_keys, const V* in_vals, K* keys, V* vals, int64_t n, h
   return RAJA_CDNA4_SUCCESS;
 }
 
-// In-place sort with the stream's scratch.
+// In-place sort with the stream's scratch, in the tuned tile shape of its
+// (key type, length bucket) (items < 0: look it up; else this shape).
 template<typename K, typename V, bool HAS_VALS, bool DESC>
-int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
+int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s, int items = -1)
 {
   if (n > kMaxSortN)
   {
     return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort length %lld exceeds %lld",
                          (long long)n, (long long)kMaxSortN);
   }
+  using S = SortShape<K, V, HAS_VALS>;
+  if (items < 0)
+  {
+    items = 0;
+    if (!S::wide) RC_TRY(rtune::choice(rtune::kSortPairs, rk::dtype_code<K>(), n, s, &items));
+  }
+  if (items != S::kChoices[1] && items != S::kChoices[2]) items = 0;
   rc::StreamCtx* c = nullptr;
   RC_TRY(rc::get_ctx(s, &c));
-  const SortWs<K, V, HAS_VALS> w(n);
+  const SortWs<K, V, HAS_VALS> w(n, items);
   RC_TRY(rc::ensure_tmp(c, w.tmp_bytes()));
   RC_TRY(rc::ensure_sort_desc(c, w.desc_bytes));
   // Under stream capture the histogram readback (a host wait) cannot run:
@@ -638,7 +770,7 @@ int sort_impl(K* keys, V* vals, int64_t n, hipStream_t s)
   RC_HIP(hipStreamIsCapturing(s, &cap));
   uint32_t* const pinned = cap == hipStreamCaptureStatusNone ? (uint32_t*)c->pinned : nullptr;
   return sort_core<K, V, HAS_VALS, DESC>(keys, vals, keys, vals, n, s, (char*)c->tmp,
-                                         (uint32_t*)c->sort_desc, pinned);
+                                         (uint32_t*)c->sort_desc, pinned, items);
 }
 
 template<typename K>
@@ -666,6 +798,26 @@ int sort_key_type(void* keys, void* vals, int val_bytes, int64_t n, int descendi
 
 namespace rsort
 {
+void tile_choices(int key_dtype, int out[3])
+{
+  const int* c = (key_dtype == RAJA_CDNA4_U32 || key_dtype == RAJA_CDNA4_I32 ||
+                  key_dtype == RAJA_CDNA4_F32)
+                     ? SortShape<uint32_t, uint32_t, true>::kChoices
+                     : SortShape<uint64_t, uint32_t, true>::kChoices;
+  for (int i = 0; i < 3; ++i) out[i] = c[i];
+}
+
+int tune_sort(int key_dtype, void* keys, void* vals, int64_t n, int items, hipStream_t s)
+{
+  switch (key_dtype)
+  {
+    case RAJA_CDNA4_U32: return sort_impl<uint32_t, uint32_t, true, false>((uint32_t*)keys, (uint32_t*)vals, n, s, items);
+    case RAJA_CDNA4_I32: return sort_impl<int32_t, uint32_t, true, false>((int32_t*)keys, (uint32_t*)vals, n, s, items);
+    case RAJA_CDNA4_F32: return sort_impl<float, uint32_t, true, false>((float*)keys, (uint32_t*)vals, n, s, items);
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "sort shapes are tuned for 4-byte keys");
+  }
+}
+
 int take_spin_timeouts(uint32_t* count)
 {
   uint32_t v = 0;
