@@ -177,6 +177,21 @@ int raja_cdna4_inclusive_scan(int dtype, int op, const void* in, void* out,
 int raja_cdna4_exclusive_scan(int dtype, int op, const void* in, void* out,
                               int64_t n, const void* init, void* stream);
 
+/* Allocation-free raja_cdna4_exclusive_scan.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Same arguments, semantics and bits as
+ * raja_cdna4_exclusive_scan, but it calls no allocator: its lookback
+ * descriptors live in a fixed 1 MiB device array that each GPU holds until
+ * the process ends (looked up by raja_cdna4_init, or by the first call on a
+ * device).  Sizes beyond that array -- more than 131072 tiles of 32768
+ * 4-byte elements, or 65536 tiles of 12288 8-byte elements (n > 805306368)
+ * -- fail with RAJA_CDNA4_ERROR_UNSUPPORTED.  Every call on one GPU shares
+ * the array, so calls that overlap on one GPU (concurrent streams, or
+ * threads) race and give wrong results: serialise them. */
+int raja_cdna4_exclusive_scan_noalloc(int dtype, int op, const void* in, void* out,
+                                      int64_t n, const void* init, void* stream);
+
 /* ---- sorts ------------------------------------------------------------------- */
 /* RAJA::impl::sort::{stable,unstable}_pairs (include/RAJA/policy/hip/sort.hpp:
  * 417-495, 653-663): sort keys ascending (descending = 0, operators::less) or

@@ -95,6 +95,7 @@ def _declare(L):
         "raja_cdna4_multi_reduce_f64": [vp, i64, vp, vp],
         "raja_cdna4_inclusive_scan": [i32, i32, vp, vp, i64, vp],
         "raja_cdna4_exclusive_scan": [i32, i32, vp, vp, i64, vp, vp],
+        "raja_cdna4_exclusive_scan_noalloc": [i32, i32, vp, vp, i64, vp, vp],
         "raja_cdna4_sort_pairs": [i32, vp, vp, i32, i64, i32, vp],
         "raja_cdna4_autotune": [i32, i32, i64, vp],
         "raja_cdna4_tune_query": [i32, i32, i64, ctypes.POINTER(i32), ctypes.POINTER(i32)],

@@ -269,6 +269,7 @@ int raja_cdna4_init(int device)
   (void)num_cus();
   StreamCtx* c = nullptr;
   RC_TRY(get_ctx(nullptr, &c));
+  RC_TRY(rscan::noalloc_setup(device));
   return RAJA_CDNA4_SUCCESS;
 }
 

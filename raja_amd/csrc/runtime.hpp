@@ -89,6 +89,9 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 namespace rscan
 {
 int take_spin_timeouts(uint32_t* count);
+// Looks up the current device's copy of the allocation-free scan's fixed
+// scratch (once per device).
+int noalloc_setup(int device);
 }
 namespace rsort
 {

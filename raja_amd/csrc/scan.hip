@@ -19,6 +19,7 @@
 //   * descriptors carry a per-stream epoch chosen by the host, so they never
 //     need clearing between calls; under graph capture (where the epoch is
 //     frozen into the graph) a clear kernel runs first.
+#include <mutex>
 #include <type_traits>
 
 #include "ops.hpp"
@@ -756,6 +757,121 @@ int table_exclusive_i32(const int32_t* in_base, int32_t* out_base, const rs::Til
 
 int64_t scan_tile_i32() { return ScanShape<int32_t, 1>::TILE; }
 }  // namespace rws
+
+// ---- allocation-free exclusive scan ---------------------------------------------
+// raja_cdna4_exclusive_scan_noalloc keeps its lookback descriptors in this
+// file-scope array instead of the stream's scratch: every GPU the library
+// runs on holds one copy of it until the process ends.  1 MiB holds 131072
+// descriptors of 4-byte values (tiles of 32768: 2^32 elements, more than
+// any int-indexed range) or 65536 of 8-byte values (tiles of 12288: about
+// 8.05e8 elements).
+namespace
+{
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
+__device__ __align__(256) uint64_t scratch[kNoAllocScratchBytes / sizeof(uint64_t)];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};  // device address of `scratch`, per GPU
+
+// Device address of this GPU's copy of `scratch`; looked up once per device
+// with that device current (the array's name on the host is only a handle).
+int noalloc_scratch(int dev, void** out)
+{
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "device %d beyond the library's %d", dev,
+                         kNoAllocMaxDevices);
+  }
+  std::lock_guard<std::mutex> g(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev])
+  {
+    void* p = nullptr;
+    RC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return rc::set_error(RAJA_CDNA4_ERROR_HIP, "no device address for the scan scratch");
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return RAJA_CDNA4_SUCCESS;
+}
+
+template<typename T, typename Op>
+int noalloc_scan_op(const T* in, T* out, int64_t n, T init, hipStream_t s, void* desc)
+{
+  const int64_t ntiles = rc::ceil_div(n, ScanShape<T>::TILE);
+  if ((size_t)ntiles * sizeof(Desc<T>) > kNoAllocScratchBytes)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED,
+                         "allocation-free scan of %lld elements needs %lld lookback descriptors; "
+                         "its fixed scratch holds %lld",
+                         (long long)n, (long long)ntiles,
+                         (long long)(kNoAllocScratchBytes / sizeof(Desc<T>)));
+  }
+  // The descriptors keep the previous call's contents: a clear kernel (no
+  // memset, so the call can also be captured into a graph) zeroes them, and
+  // the fixed tag 1 then marks this call's.  No timing events are created.
+  return launch_scan_core<T, Op, true>(in, out, n, init, s, nullptr, nullptr, ntiles, desc,
+                                       1u << 2, true);
+}
+
+template<typename T>
+int noalloc_scan_typed(int op, const void* in, void* out, int64_t n, const void* init,
+                       hipStream_t s, void* desc)
+{
+  switch (op)
+  {
+    case RAJA_CDNA4_OP_PLUS:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpPlus<T>::ident();
+      return noalloc_scan_op<T, rk::OpPlus<T>>((const T*)in, (T*)out, n, iv, s, desc);
+    }
+    case RAJA_CDNA4_OP_MIN:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpMin<T>::ident();
+      return noalloc_scan_op<T, rk::OpMin<T>>((const T*)in, (T*)out, n, iv, s, desc);
+    }
+    case RAJA_CDNA4_OP_MAX:
+    {
+      const T iv = init ? *static_cast<const T*>(init) : rk::OpMax<T>::ident();
+      return noalloc_scan_op<T, rk::OpMax<T>>((const T*)in, (T*)out, n, iv, s, desc);
+    }
+    default:
+      return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "scan operator %d not supported", op);
+  }
+}
+}  // namespace
+
+namespace rscan
+{
+int noalloc_setup(int device) { void* p = nullptr; return noalloc_scratch(device, &p); }
+}  // namespace rscan
+
+extern "C" {
+
+int raja_cdna4_exclusive_scan_noalloc(int dtype, int op, const void* in, void* out, int64_t n,
+                                      const void* init, void* stream)
+{
+  if (n < 0) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "negative length");
+  if (n == 0) return RAJA_CDNA4_SUCCESS;
+  if (!in || !out) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL span");
+  int dev = 0;
+  RC_HIP(hipGetDevice(&dev));
+  void* desc = nullptr;
+  RC_TRY(noalloc_scratch(dev, &desc));
+  hipStream_t s = (hipStream_t)stream;
+  switch (dtype)
+  {
+    case RAJA_CDNA4_I32: return noalloc_scan_typed<int32_t>(op, in, out, n, init, s, desc);
+    case RAJA_CDNA4_I64: return noalloc_scan_typed<int64_t>(op, in, out, n, init, s, desc);
+    case RAJA_CDNA4_U32: return noalloc_scan_typed<uint32_t>(op, in, out, n, init, s, desc);
+    case RAJA_CDNA4_U64: return noalloc_scan_typed<uint64_t>(op, in, out, n, init, s, desc);
+    case RAJA_CDNA4_F32: return noalloc_scan_typed<float>(op, in, out, n, init, s, desc);
+    case RAJA_CDNA4_F64: return noalloc_scan_typed<double>(op, in, out, n, init, s, desc);
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "dtype %d not supported", dtype);
+  }
+}
+
+}  // extern "C"
 
 extern "C" {
 
