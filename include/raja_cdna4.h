@@ -199,7 +199,8 @@ int raja_cdna4_exclusive_scan_noalloc(int dtype, int op, const void* in, void* o
  * radix sort is stable, so both variants give the stable result.  vals may be
  * NULL (keys only: RAJA::impl::sort::{stable,unstable}, sort.hpp:77-356).
  * val_bytes is 4 or 8.  Signed and floating keys order as operator< does
- * (-0.0 sorts before +0.0).  Keys with every digit pass constant are not
+ * (-0.0 and +0.0 are equal keys and keep their input order; NaN keys have no
+ * defined place).  Keys with every digit pass constant are not
  * re-sorted for that pass. */
 int raja_cdna4_sort_pairs(int key_dtype, void* keys, void* vals, int val_bytes,
                           int64_t n, int descending, void* stream);
@@ -352,14 +353,40 @@ typedef struct
   double done_s;      /* completion seen by the server, seconds after the run start   */
 } raja_cdna4_result;
 
+typedef struct
+{
+  int64_t ticket;           /* from raja_cdna4_server_submit                        */
+  raja_cdna4_result result; /* times in seconds after raja_cdna4_server_start       */
+} raja_cdna4_completion;
+
 typedef struct raja_cdna4_server_s* raja_cdna4_server;
 
 /* Creates the server's streams and arena on the current device. */
 int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_server* server);
 /* Runs nreq requests to completion (blocking) and fills results[0..nreq);
- * *wall_s (optional) is the time from the run start to the last completion. */
+ * *wall_s (optional) is the time from the run start to the last completion.
+ * A replay of a known request list (arrival_s honoured when the config asks);
+ * refused while the server is started. */
 int raja_cdna4_server_run(raja_cdna4_server server, const raja_cdna4_request* requests,
                           int64_t nreq, raja_cdna4_result* results, double* wall_s);
+
+/* Open serving: start the server's scheduling thread, which admits
+ * requests as they are submitted (batching those waiting together, exactly
+ * as server_run does) while earlier units run. */
+int raja_cdna4_server_start(raja_cdna4_server server);
+/* Queue n requests (copied; their buffers must stay valid until their
+ * completion is polled); tickets[i] (optional) receives request i's ticket.
+ * arrival_s is ignored: a request arrives when it is submitted.  Safe to call
+ * from several threads. */
+int raja_cdna4_server_submit(raja_cdna4_server server, const raja_cdna4_request* requests,
+                             int64_t n, int64_t* tickets);
+/* Moves up to max completions into out[0..*count), waiting up to timeout_s
+ * for the first one when none is ready and requests are outstanding. */
+int raja_cdna4_server_poll(raja_cdna4_server server, raja_cdna4_completion* out, int64_t max,
+                           double timeout_s, int64_t* count);
+/* Waits until every submitted request has completed (their completions stay
+ * to be polled), then ends the scheduling thread.  destroy stops first. */
+int raja_cdna4_server_stop(raja_cdna4_server server);
 int raja_cdna4_server_destroy(raja_cdna4_server server);
 
 #ifdef __cplusplus

@@ -80,6 +80,10 @@ class Result(ctypes.Structure):
                 ("pad", ctypes.c_int), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 
+class Completion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_int64), ("result", Result)]
+
+
 def _declare(L):
     vp, i64, i32, u64, dbl = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                               ctypes.c_uint64, ctypes.c_double)
@@ -117,6 +121,10 @@ def _declare(L):
         "raja_cdna4_server_run": [vp, ctypes.POINTER(Request), i64, ctypes.POINTER(Result),
                                   ctypes.POINTER(dbl)],
         "raja_cdna4_server_destroy": [vp],
+        "raja_cdna4_server_start": [vp],
+        "raja_cdna4_server_submit": [vp, ctypes.POINTER(Request), i64, ctypes.POINTER(i64)],
+        "raja_cdna4_server_poll": [vp, ctypes.POINTER(Completion), i64, dbl, ctypes.POINTER(i64)],
+        "raja_cdna4_server_stop": [vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -414,6 +422,37 @@ class Server:
         torch.cuda.synchronize()  # inputs written by torch on its streams are visible
         _check(lib().raja_cdna4_server_run(self.handle, requests, n, results, ctypes.byref(wall)))
         return results, wall.value
+
+    # ---- open serving: a scheduling thread admits requests as they come ----
+
+    def start(self):
+        """Start the server's scheduling thread (raja_cdna4_server_start)."""
+        _check(lib().raja_cdna4_server_start(self.handle))
+
+    def submit(self, requests):
+        """Queue requests (a Request, a list or a ctypes array); returns their
+        tickets.  Their buffers must stay alive until they complete."""
+        if isinstance(requests, Request):
+            requests = [requests]
+        if not isinstance(requests, ctypes.Array):
+            requests = (Request * len(requests))(*requests)
+        n = len(requests)
+        tickets = (ctypes.c_int64 * max(n, 1))()
+        _check(lib().raja_cdna4_server_submit(self.handle, requests, n, tickets))
+        return list(tickets[:n])
+
+    def poll(self, max_count=1024, timeout_s=0.0):
+        """Completions ready now (or within timeout_s): a list of
+        (ticket, Result)."""
+        out = (Completion * max(max_count, 1))()
+        cnt = ctypes.c_int64(0)
+        _check(lib().raja_cdna4_server_poll(self.handle, out, max_count, float(timeout_s),
+                                             ctypes.byref(cnt)))
+        return [(out[i].ticket, out[i].result) for i in range(cnt.value)]
+
+    def stop(self):
+        """Wait for every submitted request, then end the scheduling thread."""
+        _check(lib().raja_cdna4_server_stop(self.handle))
 
     def close(self):
         if self.handle:

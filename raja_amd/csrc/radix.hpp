@@ -30,7 +30,8 @@ struct KeyTraits<float>
   using bits_t = uint32_t;
   __device__ static bits_t to_bits(float k)
   {
-    const uint32_t u = __float_as_uint(k);
+    uint32_t u = __float_as_uint(k);
+    u = (u == 0x80000000u) ? 0u : u;  // -0.0 == +0.0 under operator<: one key
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
   }
 };
@@ -52,7 +53,8 @@ struct KeyTraits<double>
   using bits_t = uint64_t;
   __device__ static bits_t to_bits(double k)
   {
-    const uint64_t u = (uint64_t)__double_as_longlong(k);
+    uint64_t u = (uint64_t)__double_as_longlong(k);
+    u = (u == 0x8000000000000000ull) ? 0ull : u;  // -0.0 == +0.0 under operator<
     return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
   }
 };

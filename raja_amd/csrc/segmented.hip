@@ -15,8 +15,11 @@
 //     Segments of at most one wave chunk are scanned one per wave by a
 //     second kernel on the same stream (no descriptors, same bits);
 //   * sort_pairs: one block per segment of up to 8192 keys sorts it in LDS
-//     (LSD passes with the onesweep kernel's stable wave ranking); longer
-//     segments go through the onesweep sort on the same stream.
+//     (LSD passes with the onesweep kernel's stable wave ranking); the
+//     longer segments of 4-byte keys (and 4-byte or no values) are sorted
+//     together by the segmented onesweep (one histogram launch, one launch
+//     per digit pass over all their tiles); 8-byte keys or values take one
+//     onesweep sort per longer segment.
 #include <algorithm>
 #include <vector>
 
@@ -315,13 +318,62 @@ int seg_sort(raja_cdna4_segments_s* p, K* keys, V* vals, int key_dtype, int val_
                        dim3(S::BLOCK), 0, s, keys, vals, keys, vals, (const rs::SortSeg*)d_segs);
     RC_HIP(hipGetLastError());
   }
-  for (int64_t g : big)
+  if (big.empty()) return RAJA_CDNA4_SUCCESS;
+  if constexpr (sizeof(K) == 4 && (!HAS_VALS || sizeof(V) == 4))
   {
-    const int64_t b = p->offsets[(size_t)g], len = p->offsets[(size_t)g + 1] - b;
-    RC_TRY(raja_cdna4_sort_pairs(key_dtype, keys + b, HAS_VALS ? (void*)(vals + b) : nullptr,
-                                 val_bytes, len, DESC ? 1 : 0, s));
+    // Every longer segment in one segmented onesweep: tile records built
+    // once per plan; alternates, histograms and descriptors from the
+    // stream's scratch.
+    const int64_t tile = rsort::seg_tile();
+    raja_cdna4_segments_s::SortTable tab;
+    {
+      std::lock_guard<std::mutex> g(p->lock);
+      auto it = p->sort_tables.find((int64_t)S::TILE);
+      if (it == p->sort_tables.end())
+      {
+        std::vector<rs::SortTile> recs;
+        for (size_t r = 0; r < big.size(); ++r)
+        {
+          const int64_t b = p->offsets[(size_t)big[r]];
+          const int64_t len = p->offsets[(size_t)big[r] + 1] - b;
+          for (int64_t j = 0; j * tile < len; ++j)
+          {
+            recs.push_back(rs::SortTile{b, b, b, b, b, b, len, (uint32_t)j, (uint32_t)r});
+          }
+        }
+        raja_cdna4_segments_s::SortTable t;
+        t.ntiles = (int64_t)recs.size();
+        t.nranges = (int64_t)big.size();
+        RC_HIP(hipMalloc(&t.d_tiles, sizeof(rs::SortTile) * recs.size()));
+        RC_HIP(hipMemcpy(t.d_tiles, recs.data(), sizeof(rs::SortTile) * recs.size(),
+                         hipMemcpyHostToDevice));
+        it = p->sort_tables.emplace((int64_t)S::TILE, t).first;
+      }
+      tab = it->second;
+    }
+    rc::StreamCtx* c = nullptr;
+    RC_TRY(rc::get_ctx(s, &c));
+    const size_t kb = rc::align_up((size_t)p->total * sizeof(K), 256);
+    const size_t vb = HAS_VALS ? rc::align_up((size_t)p->total * sizeof(V), 256) : 0;
+    RC_TRY(rc::ensure_tmp(c, kb + vb + rsort::seg_hist_bytes(tab.nranges)));
+    RC_TRY(rc::ensure_sort_desc(c, rsort::seg_desc_bytes(tab.ntiles)));
+    char* tmp = (char*)c->tmp;
+    return rsort::seg_sort_table(key_dtype, HAS_VALS ? 4 : 0, DESC ? 1 : 0, keys, vals, keys, vals,
+                                 tmp, HAS_VALS ? (void*)(tmp + kb) : nullptr, tab.d_tiles,
+                                 tab.ntiles, tab.nranges, (uint32_t*)(tmp + kb + vb),
+                                 (uint32_t*)c->sort_desc, s);
   }
-  return RAJA_CDNA4_SUCCESS;
+  else
+  {
+    // 8-byte keys or values: one onesweep sort per longer segment.
+    for (int64_t g : big)
+    {
+      const int64_t b = p->offsets[(size_t)g], len = p->offsets[(size_t)g + 1] - b;
+      RC_TRY(raja_cdna4_sort_pairs(key_dtype, keys + b, HAS_VALS ? (void*)(vals + b) : nullptr,
+                                   val_bytes, len, DESC ? 1 : 0, s));
+    }
+    return RAJA_CDNA4_SUCCESS;
+  }
 }
 
 template<typename K>
@@ -433,6 +485,7 @@ int raja_cdna4_segments_destroy(raja_cdna4_segments_s* p)
     if (kv.second.d_short) RC_HIP(hipFree(kv.second.d_short));
   }
   for (auto& kv : p->sort_segs) RC_HIP(hipFree(kv.second));
+  for (auto& kv : p->sort_tables) RC_HIP(hipFree(kv.second.d_tiles));
   if (p->d_offsets) RC_HIP(hipFree(p->d_offsets));
   delete p;
   return RAJA_CDNA4_SUCCESS;

@@ -85,6 +85,15 @@ struct raja_cdna4_segments_s
   int64_t* d_offsets = nullptr;
   std::map<std::pair<int64_t, int64_t>, rs::TileTable> tables;  // (tile, short limit)
   std::map<int64_t, rs::SortSeg*> sort_segs;  // per block shape: short-segment records
+  // tile records of the segments longer than one block sort (segmented
+  // onesweep), per block-sort limit
+  struct SortTable
+  {
+    rs::SortTile* d_tiles = nullptr;
+    int64_t ntiles = 0;
+    int64_t nranges = 0;
+  };
+  std::map<int64_t, SortTable> sort_tables;
   std::mutex lock;
 };
 
@@ -94,6 +103,22 @@ namespace rs
 // short_max > 0, segments of at most short_max elements go to the short list.
 int get_table(raja_cdna4_segments_s* p, int64_t tile, const TileTable** out, int64_t short_max = 0);
 }  // namespace rs
+
+// Segmented onesweep (sort.hip): every range of a tile table of SortTile
+// records (tile = seg_tile() keys, 4-byte keys, values of 4 bytes or none)
+// sorted in one histogram launch and one launch per digit pass, from the
+// input through the alternates into the output buffers.  hist holds
+// seg_hist_bytes(nranges), desc seg_desc_bytes(ntiles); neither needs
+// clearing.
+namespace rsort
+{
+int64_t seg_tile();
+size_t seg_desc_bytes(int64_t ntiles);
+size_t seg_hist_bytes(int64_t nranges);
+int seg_sort_table(int key_dtype, int val_bytes, int descending, const void* in_k, const void* in_v,
+                   void* out_k, void* out_v, void* alt_k, void* alt_v, const rs::SortTile* d_recs,
+                   int64_t ntiles, int64_t nranges, uint32_t* hist, uint32_t* desc, hipStream_t s);
+}  // namespace rsort
 
 // Launcher implemented in scan.hip for segmented scans (SEG mode of the
 // single-pass scan kernel; lookback stops at each segment's first tile).

@@ -13,16 +13,19 @@
 //     each kind with one table upload and one or two kernels: sums over the
 //     tile table of the canonical reduction tree (results land in each
 //     request's own output), scans with the segmented single-pass scan
-//     (tile records carry each request's output offset), sorts with one
-//     block per request in LDS;
+//     (tile records carry each request's output offset), sorts of up to
+//     8192 pairs with one block per request in LDS and longer ones with the
+//     segmented onesweep (one launch per digit pass over all their tiles);
 //   * completion is observed by polling one event per unit, so new arrivals
 //     are admitted while earlier units run.
 #include <string.h>
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -93,18 +96,31 @@ struct Slot
   size_t staging_cap = 0;
 };
 
+// A request waiting for admission: its ticket (run(): its index) and the
+// time it entered the queue.
+struct Pending
+{
+  raja_cdna4_request req;
+  int64_t ticket = 0;
+  double arrival = 0.0;
+};
+
 struct Unit
 {
   int slot = -1;
   int id = 0;
   size_t off = 0, bytes = 0;
-  std::vector<int64_t> reqs;
+  std::vector<int64_t> tickets;
+  double start = 0.0;
 };
 
 bool is_batchable(const raja_cdna4_serve_config& cfg, const raja_cdna4_request& r)
 {
   if (cfg.mode != RAJA_CDNA4_SERVE_BATCHED) return false;
-  if (r.kind == RAJA_CDNA4_REQ_SORT_PAIRS_U32_I32) return r.n <= rws::block_sort_max();
+  if (r.kind == RAJA_CDNA4_REQ_SORT_PAIRS_U32_I32)
+  {
+    return r.n <= rws::block_sort_max() || r.n <= cfg.batch_limit;
+  }
   return r.n <= cfg.batch_limit;
 }
 
@@ -135,10 +151,18 @@ size_t batch_bytes_bound(const raja_cdna4_request& r)
       const size_t t = (size_t)rc::ceil_div(r.n, rws::scan_tile_i32());
       return t * (sizeof(rs::TileRec) + 16);
     }
-    default: return sizeof(rs::SortSeg);
+    default:
+    {
+      if (r.n <= rws::block_sort_max()) return sizeof(rs::SortSeg);
+      // segmented onesweep: records, descriptors, a histogram row and the
+      // alternate keys and values
+      const int64_t t = rc::ceil_div(r.n, rsort::seg_tile());
+      return (size_t)t * sizeof(rs::SortTile) + rsort::seg_desc_bytes(t) + rsort::seg_hist_bytes(1) +
+             2 * rc::align_up((size_t)r.n * 4, kAlign);
+    }
   }
 }
-constexpr size_t kBatchFixedBound = 6 * 16 + 4 * kAlign + 16;
+constexpr size_t kBatchFixedBound = 6 * 16 + 8 * kAlign + 16;
 
 // Host image of one batch: the records of each kind plus the layout of the
 // device copy (records first, then partials and descriptors).
@@ -147,10 +171,13 @@ struct BatchPlan
   std::vector<rs::TileRec> sum_tiles, scan_tiles;
   std::vector<uint32_t> sum_first, sum_count;
   std::vector<int64_t> sum_out;
-  std::vector<rs::SortSeg> sorts;
+  std::vector<rs::SortSeg> sorts;       // one block each (block_sort_max() keys or fewer)
+  std::vector<rs::SortTile> sort_tiles;  // tiles of the longer sorts (segmented onesweep)
+  int64_t sort_ranges = 0, alt_elems = 0;
   uintptr_t sum_base = 0, scan_base = 0, sort_base = 0;
   size_t o_sum_tiles = 0, o_sum_first = 0, o_sum_count = 0, o_sum_out = 0, o_scan_tiles = 0,
-         o_sorts = 0, rec_bytes = 0, o_partials = 0, o_desc = 0, total = 0;
+         o_sorts = 0, o_sort_tiles = 0, rec_bytes = 0, o_partials = 0, o_desc = 0, o_sdesc = 0,
+         o_hist = 0, o_alt_k = 0, o_alt_v = 0, total = 0;
 
   void clear()
   {
@@ -160,6 +187,8 @@ struct BatchPlan
     sum_count.clear();
     sum_out.clear();
     sorts.clear();
+    sort_tiles.clear();
+    sort_ranges = alt_elems = 0;
   }
 
   static uintptr_t lowest(uintptr_t a, const void* p)
@@ -168,14 +197,14 @@ struct BatchPlan
     return (a == 0 || v < a) ? v : a;
   }
 
-  // Builds the records of the requests `ids` (all batchable).
-  void build(const raja_cdna4_request* req, const std::vector<int64_t>& ids)
+  // Builds the records of the requests `reqs` (all batchable).
+  void build(const std::vector<const raja_cdna4_request*>& reqs)
   {
     clear();
     sum_base = scan_base = sort_base = 0;
-    for (int64_t i : ids)
+    for (const raja_cdna4_request* rp : reqs)
     {
-      const raja_cdna4_request& r = req[i];
+      const raja_cdna4_request& r = *rp;
       if (r.kind == RAJA_CDNA4_REQ_SUM_F64)
       {
         sum_base = lowest(lowest(sum_base, r.out), r.n ? r.in : r.out);
@@ -191,9 +220,9 @@ struct BatchPlan
     }
     const int64_t sum_tile = rws::sum_tile_f64();
     const int64_t scan_tile = rws::scan_tile_i32();
-    for (int64_t i : ids)
+    for (const raja_cdna4_request* rp : reqs)
     {
-      const raja_cdna4_request& r = req[i];
+      const raja_cdna4_request& r = *rp;
       if (r.kind == RAJA_CDNA4_REQ_SUM_F64)
       {
         const int64_t start = r.n ? (int64_t)(((uintptr_t)r.in - sum_base) / 8) : 0;
@@ -210,12 +239,29 @@ struct BatchPlan
         const int64_t ostart = (int64_t)(((uintptr_t)r.out - scan_base) / 4);
         rs::append_tiles(scan_tiles, start, r.n, ostart, (uint32_t)(scan_tiles.size()), scan_tile);
       }
-      else if (r.n)
+      else if (r.n && r.n <= rws::block_sort_max())
       {
         sorts.push_back(rs::SortSeg{(int64_t)(((uintptr_t)r.in - sort_base) / 4),
                                     (int64_t)(((uintptr_t)r.in_vals - sort_base) / 4),
                                     (int64_t)(((uintptr_t)r.out - sort_base) / 4),
                                     (int64_t)(((uintptr_t)r.out_vals - sort_base) / 4), r.n});
+      }
+      else if (r.n)
+      {
+        // offsets from sort_base (inputs, outputs) and from the batch's
+        // alternate buffers (ak = av)
+        const int64_t ik = (int64_t)(((uintptr_t)r.in - sort_base) / 4);
+        const int64_t iv = (int64_t)(((uintptr_t)r.in_vals - sort_base) / 4);
+        const int64_t ok = (int64_t)(((uintptr_t)r.out - sort_base) / 4);
+        const int64_t ov = (int64_t)(((uintptr_t)r.out_vals - sort_base) / 4);
+        const int64_t tile = rsort::seg_tile();
+        for (int64_t j = 0; j * tile < r.n; ++j)
+        {
+          sort_tiles.push_back(rs::SortTile{ik, iv, ok, ov, alt_elems, alt_elems, r.n, (uint32_t)j,
+                                            (uint32_t)sort_ranges});
+        }
+        ++sort_ranges;
+        alt_elems += (r.n + 63) / 64 * 64;
       }
     }
     size_t o = 0;
@@ -227,6 +273,7 @@ struct BatchPlan
     o_sum_tiles = place(sizeof(rs::TileRec) * sum_tiles.size());
     o_scan_tiles = place(sizeof(rs::TileRec) * scan_tiles.size());
     o_sorts = place(sizeof(rs::SortSeg) * sorts.size());
+    o_sort_tiles = place(sizeof(rs::SortTile) * sort_tiles.size());
     o_sum_out = place(sizeof(int64_t) * sum_out.size());
     o_sum_first = place(sizeof(uint32_t) * sum_first.size());
     o_sum_count = place(sizeof(uint32_t) * sum_count.size());
@@ -236,6 +283,14 @@ struct BatchPlan
     o += rc::align_up(sizeof(double) * (sum_tiles.size() + 1), kAlign);
     o_desc = o;
     o += rws::exclusive_i32_ws_bytes((int64_t)scan_tiles.size(), true);
+    o_sdesc = o;
+    o += rc::align_up(rsort::seg_desc_bytes((int64_t)sort_tiles.size()), kAlign);
+    o_hist = o;
+    o += rc::align_up(rsort::seg_hist_bytes(sort_ranges), kAlign);
+    o_alt_k = o;
+    o += rc::align_up((size_t)alt_elems * 4, kAlign);
+    o_alt_v = o;
+    o += rc::align_up((size_t)alt_elems * 4, kAlign);
     total = o;
   }
 
@@ -244,6 +299,7 @@ struct BatchPlan
     ::memcpy(dst + o_sum_tiles, sum_tiles.data(), sizeof(rs::TileRec) * sum_tiles.size());
     ::memcpy(dst + o_scan_tiles, scan_tiles.data(), sizeof(rs::TileRec) * scan_tiles.size());
     ::memcpy(dst + o_sorts, sorts.data(), sizeof(rs::SortSeg) * sorts.size());
+    ::memcpy(dst + o_sort_tiles, sort_tiles.data(), sizeof(rs::SortTile) * sort_tiles.size());
     ::memcpy(dst + o_sum_out, sum_out.data(), sizeof(int64_t) * sum_out.size());
     ::memcpy(dst + o_sum_first, sum_first.data(), sizeof(uint32_t) * sum_first.size());
     ::memcpy(dst + o_sum_count, sum_count.data(), sizeof(uint32_t) * sum_count.size());
@@ -285,6 +341,19 @@ struct raja_cdna4_server_s
   std::vector<hipStream_t> streams;
   std::vector<Slot> slots;
   BatchPlan plan;
+
+  // Open serving (raja_cdna4_server_start / submit / poll / stop): the
+  // scheduling thread takes submitted requests from `inbox` and posts
+  // completions to `outbox`.
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_in, cv_out;
+  std::deque<Pending> inbox;
+  std::deque<raja_cdna4_completion> outbox;
+  int64_t next_ticket = 0;
+  int64_t outstanding = 0;  // submitted, not yet posted to outbox
+  bool running = false, stopping = false;
+  Clock::time_point t0;
 };
 
 namespace
@@ -339,6 +408,14 @@ int launch_batch(raja_cdna4_server_s* sv, Slot& slot, char* dev, hipStream_t s)
     RC_TRY(rws::block_sort_u32_i32((const uint32_t*)p.sort_base, (const rs::SortSeg*)(dev + p.o_sorts),
                                    (int64_t)p.sorts.size(), s));
   }
+  if (!p.sort_tiles.empty())
+  {
+    void* base = (void*)p.sort_base;
+    RC_TRY(rsort::seg_sort_table(RAJA_CDNA4_U32, 4, 0, base, base, base, base, dev + p.o_alt_k,
+                                 dev + p.o_alt_v, (const rs::SortTile*)(dev + p.o_sort_tiles),
+                                 (int64_t)p.sort_tiles.size(), p.sort_ranges,
+                                 (uint32_t*)(dev + p.o_hist), (uint32_t*)(dev + p.o_sdesc), s));
+  }
   return RAJA_CDNA4_SUCCESS;
 }
 
@@ -347,14 +424,156 @@ double seconds_since(Clock::time_point t0)
   return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-// Waits for every unit in flight (error path) and returns their memory.
-void drain(raja_cdna4_server_s* sv, std::deque<Unit>& inflight)
+// Admission and retirement over a queue of ready requests, shared by the
+// closed replay (run) and the open server's thread.  `finish(ticket,
+// result)` reports each request once.
+struct Scheduler
 {
-  for (hipStream_t s : sv->streams) (void)hipStreamSynchronize(s);
-  for (Unit& u : inflight) sv->arena.release(u.off, u.bytes);
-  inflight.clear();
-}
+  raja_cdna4_server_s* sv;
+  std::deque<Pending> ready;
+  std::deque<Unit> inflight;
+  std::vector<int> free_slots;
+  int nstreams = 1, next_stream = 0, units = 0;
+  Clock::time_point t0;
 
+  Scheduler(raja_cdna4_server_s* s, Clock::time_point start) : sv(s), t0(start)
+  {
+    for (int k = (int)sv->slots.size() - 1; k >= 0; --k) free_slots.push_back(k);
+    nstreams = sv->cfg.mode == RAJA_CDNA4_SERVE_BATCHED ? (int)sv->streams.size() : 1;
+  }
+
+  bool idle() const { return ready.empty() && inflight.empty(); }
+
+  template<typename Finish>
+  void fail_unit(const Unit& u, int status, Finish& finish)
+  {
+    const double t = seconds_since(t0);
+    for (int64_t tk : u.tickets)
+    {
+      finish(tk, raja_cdna4_result{status, u.id, (int)u.tickets.size(), 0, u.start, t});
+    }
+  }
+
+  // Retires finished units; returns whether any did, or an error status
+  // (then every unit in flight has been waited for and failed).
+  template<typename Finish>
+  int retire(Finish& finish, bool* progress)
+  {
+    for (auto it = inflight.begin(); it != inflight.end();)
+    {
+      const hipError_t q = hipEventQuery(sv->slots[(size_t)it->slot].done);
+      if (q == hipErrorNotReady)
+      {
+        ++it;
+        continue;
+      }
+      if (q != hipSuccess)
+      {
+        const int st = rc::hip_error(q, "hipEventQuery", __FILE__, __LINE__);
+        for (hipStream_t s : sv->streams) (void)hipStreamSynchronize(s);
+        for (Unit& u : inflight)
+        {
+          fail_unit(u, st, finish);
+          sv->arena.release(u.off, u.bytes);
+        }
+        inflight.clear();
+        return st;
+      }
+      const double t = seconds_since(t0);
+      for (int64_t tk : it->tickets)
+      {
+        finish(tk, raja_cdna4_result{RAJA_CDNA4_SUCCESS, it->id, (int)it->tickets.size(), 0,
+                                     it->start, t});
+      }
+      sv->arena.release(it->off, it->bytes);
+      free_slots.push_back(it->slot);
+      it = inflight.erase(it);
+      *progress = true;
+    }
+    return RAJA_CDNA4_SUCCESS;
+  }
+
+  // Launches units from the front of `ready` while slots and arena memory
+  // last: a request that is not batchable alone, else the longest run of
+  // batchable requests within max_batch and the arena.
+  template<typename Finish>
+  int admit(Finish& finish, bool* progress)
+  {
+    const raja_cdna4_serve_config& cfg = sv->cfg;
+    std::vector<const raja_cdna4_request*> reqs;
+    while (!ready.empty() && !free_slots.empty())
+    {
+      const double now = seconds_since(t0);
+      const Pending& front = ready.front();
+      size_t take = 1, bytes = 0;
+      const bool batch = is_batchable(cfg, front.req);
+      if (!batch)
+      {
+        bytes = single_ws(front.req);
+      }
+      else
+      {
+        size_t bound = kBatchFixedBound;
+        reqs.clear();
+        for (size_t j = 0; j < ready.size() && (int64_t)reqs.size() < cfg.max_batch &&
+                           is_batchable(cfg, ready[j].req);
+             ++j)
+        {
+          bound += batch_bytes_bound(ready[j].req);
+          if (bound > sv->arena.size && !reqs.empty()) break;
+          reqs.push_back(&ready[j].req);
+        }
+        take = reqs.size();
+        sv->plan.build(reqs);
+        bytes = sv->plan.total;
+      }
+      if (bytes > sv->arena.size)
+      {
+        finish(front.ticket, raja_cdna4_result{RAJA_CDNA4_ERROR_ADMISSION, -1, 0, 0, now, now});
+        ready.pop_front();
+        *progress = true;
+        continue;
+      }
+      size_t off = 0;
+      if (!sv->arena.alloc(bytes, &off)) break;  // wait for memory to come back
+      const int slot = free_slots.back();
+      free_slots.pop_back();
+      hipStream_t s = sv->streams[(size_t)(next_stream % nstreams)];
+      next_stream = (next_stream + 1) % nstreams;
+      char* dev = sv->arena.base + off;
+      Unit u;
+      u.slot = slot;
+      u.id = units++;
+      u.off = off;
+      u.bytes = bytes;
+      u.start = now;
+      for (size_t j = 0; j < take; ++j) u.tickets.push_back(ready[j].ticket);
+      int st = batch ? launch_batch(sv, sv->slots[(size_t)slot], dev, s)
+                     : launch_single(front.req, dev, s);
+      if (st == RAJA_CDNA4_SUCCESS)
+      {
+        const hipError_t e = hipEventRecord(sv->slots[(size_t)slot].done, s);
+        if (e != hipSuccess) st = rc::hip_error(e, "hipEventRecord", __FILE__, __LINE__);
+      }
+      for (size_t j = 0; j < take; ++j) ready.pop_front();
+      *progress = true;
+      if (st != RAJA_CDNA4_SUCCESS)
+      {
+        (void)hipStreamSynchronize(s);
+        fail_unit(u, st, finish);
+        sv->arena.release(off, bytes);
+        free_slots.push_back(slot);
+        return st;
+      }
+      inflight.push_back(std::move(u));
+    }
+    return RAJA_CDNA4_SUCCESS;
+  }
+};
+
+// The closed replay: requests enter the ready queue at their arrival times
+// (all at once without honor_arrivals) and the call returns when every one
+// completed.
 int run(raja_cdna4_server_s* sv, const raja_cdna4_request* req, int64_t nreq,
         raja_cdna4_result* res, double* wall_s)
 {
@@ -370,133 +589,72 @@ int run(raja_cdna4_server_s* sv, const raja_cdna4_request* req, int64_t nreq,
   }
   RC_HIP(hipSetDevice(sv->device));
   sv->arena.reset();
-  std::deque<Unit> inflight;
-  std::vector<int> free_slots;
-  for (int k = (int)sv->slots.size() - 1; k >= 0; --k) free_slots.push_back(k);
-  const int nstreams = cfg.mode == RAJA_CDNA4_SERVE_BATCHED ? (int)sv->streams.size() : 1;
-  int next_stream = 0;
-  int units = 0;
+  Scheduler sch(sv, Clock::now());
   int64_t next = 0, completed = 0;
   double last_done = 0.0;
-  const Clock::time_point t0 = Clock::now();
-  std::vector<int64_t> ids;
+  int first_error = RAJA_CDNA4_SUCCESS;
+  auto finish = [&](int64_t tk, const raja_cdna4_result& r) {
+    res[tk] = r;
+    last_done = std::max(last_done, r.done_s);
+    ++completed;
+  };
   while (completed < nreq)
   {
     bool progress = false;
-    // Retire finished units (any order: units on different streams finish
-    // independently).
-    for (auto it = inflight.begin(); it != inflight.end();)
+    const double now = seconds_since(sch.t0);
+    while (next < nreq && (!cfg.honor_arrivals || req[next].arrival_s <= now))
     {
-      const hipError_t q = hipEventQuery(sv->slots[(size_t)it->slot].done);
-      if (q == hipErrorNotReady)
-      {
-        ++it;
-        continue;
-      }
-      if (q != hipSuccess)
-      {
-        drain(sv, inflight);
-        return rc::hip_error(q, "hipEventQuery", __FILE__, __LINE__);
-      }
-      const double t = seconds_since(t0);
-      for (int64_t i : it->reqs) res[i].done_s = t;
-      last_done = std::max(last_done, t);
-      completed += (int64_t)it->reqs.size();
-      sv->arena.release(it->off, it->bytes);
-      free_slots.push_back(it->slot);
-      it = inflight.erase(it);
-      progress = true;
+      sch.ready.push_back(Pending{req[next], next, req[next].arrival_s});
+      ++next;
     }
-    // Admit waiting requests in arrival order.
-    while (next < nreq && !free_slots.empty())
+    int st = sch.retire(finish, &progress);
+    if (st == RAJA_CDNA4_SUCCESS) st = sch.admit(finish, &progress);
+    if (st != RAJA_CDNA4_SUCCESS)
     {
-      const double now = seconds_since(t0);
-      if (cfg.honor_arrivals && req[next].arrival_s > now) break;
-      ids.clear();
-      size_t bytes = 0;
-      const bool batch = is_batchable(cfg, req[next]);
-      if (!batch)
-      {
-        bytes = single_ws(req[next]);
-        if (bytes > sv->arena.size)
-        {
-          res[next].status = RAJA_CDNA4_ERROR_ADMISSION;
-          res[next].start_s = res[next].done_s = now;
-          ++next;
-          ++completed;
-          progress = true;
-          continue;
-        }
-        ids.push_back(next);
-      }
-      else
-      {
-        // Waiting batchable requests, in order, while the batch's bound stays
-        // within the arena and max_batch.
-        size_t bound = kBatchFixedBound;
-        for (int64_t j = next; j < nreq && (int64_t)ids.size() < cfg.max_batch &&
-                               is_batchable(cfg, req[j]) &&
-                               (!cfg.honor_arrivals || req[j].arrival_s <= now);
-             ++j)
-        {
-          bound += batch_bytes_bound(req[j]);
-          if (bound > sv->arena.size && !ids.empty()) break;
-          ids.push_back(j);
-        }
-        sv->plan.build(req, ids);
-        bytes = sv->plan.total;
-        if (bytes > sv->arena.size)
-        {
-          res[next].status = RAJA_CDNA4_ERROR_ADMISSION;
-          res[next].start_s = res[next].done_s = now;
-          ++next;
-          ++completed;
-          progress = true;
-          continue;
-        }
-      }
-      size_t off = 0;
-      if (!sv->arena.alloc(bytes, &off)) break;  // wait for memory to come back
-      const int slot = free_slots.back();
-      free_slots.pop_back();
-      hipStream_t s = sv->streams[(size_t)(next_stream % nstreams)];
-      next_stream = (next_stream + 1) % nstreams;
-      char* dev = sv->arena.base + off;
-      const int st = batch ? launch_batch(sv, sv->slots[(size_t)slot], dev, s)
-                           : launch_single(req[next], dev, s);
-      if (st != RAJA_CDNA4_SUCCESS)
-      {
-        sv->arena.release(off, bytes);
-        drain(sv, inflight);
-        return st;
-      }
-      const hipError_t e = hipEventRecord(sv->slots[(size_t)slot].done, s);
-      if (e != hipSuccess)
-      {
-        sv->arena.release(off, bytes);
-        drain(sv, inflight);
-        return rc::hip_error(e, "hipEventRecord", __FILE__, __LINE__);
-      }
-      Unit u;
-      u.slot = slot;
-      u.id = units++;
-      u.off = off;
-      u.bytes = bytes;
-      u.reqs = ids;
-      for (int64_t i : ids)
-      {
-        res[i].unit = u.id;
-        res[i].unit_size = (int)ids.size();
-        res[i].start_s = now;
-      }
-      inflight.push_back(std::move(u));
-      next += (int64_t)ids.size();
-      progress = true;
+      // wait for what is in flight and report the first failure
+      first_error = st;
+      for (hipStream_t s : sv->streams) (void)hipStreamSynchronize(s);
+      bool p2 = false;
+      (void)sch.retire(finish, &p2);
+      break;
     }
     if (!progress) std::this_thread::yield();
   }
   if (wall_s) *wall_s = last_done;
-  return RAJA_CDNA4_SUCCESS;
+  return first_error;
+}
+
+// The open server's scheduling thread.
+void serve_loop(raja_cdna4_server_s* sv)
+{
+  (void)hipSetDevice(sv->device);
+  Scheduler sch(sv, sv->t0);
+  auto finish = [&](int64_t tk, const raja_cdna4_result& r) {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->outbox.push_back(raja_cdna4_completion{tk, r});
+    --sv->outstanding;
+    sv->cv_out.notify_all();
+  };
+  while (true)
+  {
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      if (sch.idle() && sv->inbox.empty())
+      {
+        if (sv->stopping) break;
+        sv->cv_in.wait_for(lk, std::chrono::milliseconds(1));
+      }
+      while (!sv->inbox.empty())
+      {
+        sch.ready.push_back(sv->inbox.front());
+        sv->inbox.pop_front();
+      }
+    }
+    bool progress = false;
+    (void)sch.retire(finish, &progress);  // failures are reported per request
+    (void)sch.admit(finish, &progress);
+    if (!progress && !sch.idle()) std::this_thread::yield();
+  }
 }
 
 }  // namespace
@@ -569,12 +727,105 @@ int raja_cdna4_server_run(raja_cdna4_server_s* sv, const raja_cdna4_request* req
     return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad server run arguments");
   }
   if (wall_s) *wall_s = 0.0;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (sv->running)
+    {
+      return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "server is serving submitted requests");
+    }
+  }
   return run(sv, requests, nreq, results, wall_s);
+}
+
+int raja_cdna4_server_start(raja_cdna4_server_s* sv)
+{
+  if (!sv) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL server");
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (sv->running) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "server already started");
+  sv->arena.reset();
+  sv->inbox.clear();
+  sv->outbox.clear();
+  sv->outstanding = 0;
+  sv->stopping = false;
+  sv->t0 = Clock::now();
+  sv->running = true;
+  sv->worker = std::thread(serve_loop, sv);
+  return RAJA_CDNA4_SUCCESS;
+}
+
+int raja_cdna4_server_submit(raja_cdna4_server_s* sv, const raja_cdna4_request* reqs, int64_t n,
+                             int64_t* tickets)
+{
+  if (!sv || n < 0 || (n > 0 && !reqs))
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad submit arguments");
+  }
+  for (int64_t i = 0; i < n; ++i) RC_TRY(check_request(reqs[i]));
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (!sv->running || sv->stopping)
+    {
+      return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "server not started");
+    }
+    const double now = seconds_since(sv->t0);
+    for (int64_t i = 0; i < n; ++i)
+    {
+      Pending p;
+      p.req = reqs[i];
+      p.req.arrival_s = now;
+      p.ticket = sv->next_ticket++;
+      p.arrival = now;
+      if (tickets) tickets[i] = p.ticket;
+      sv->inbox.push_back(p);
+    }
+    sv->outstanding += n;
+  }
+  sv->cv_in.notify_one();
+  return RAJA_CDNA4_SUCCESS;
+}
+
+int raja_cdna4_server_poll(raja_cdna4_server_s* sv, raja_cdna4_completion* out, int64_t max,
+                           double timeout_s, int64_t* count)
+{
+  if (!sv || max < 0 || (max > 0 && !out) || !count)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad poll arguments");
+  }
+  std::unique_lock<std::mutex> lk(sv->mu);
+  if (sv->outbox.empty() && timeout_s > 0.0 && sv->outstanding > 0)
+  {
+    sv->cv_out.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                        [&] { return !sv->outbox.empty() || sv->outstanding == 0; });
+  }
+  int64_t k = 0;
+  while (k < max && !sv->outbox.empty())
+  {
+    out[k++] = sv->outbox.front();
+    sv->outbox.pop_front();
+  }
+  *count = k;
+  return RAJA_CDNA4_SUCCESS;
+}
+
+int raja_cdna4_server_stop(raja_cdna4_server_s* sv)
+{
+  if (!sv) return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "NULL server");
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (!sv->running) return RAJA_CDNA4_SUCCESS;
+    sv->stopping = true;
+  }
+  sv->cv_in.notify_one();
+  sv->worker.join();  // the thread ends once every submitted request completed
+  std::lock_guard<std::mutex> g(sv->mu);
+  sv->running = false;
+  return RAJA_CDNA4_SUCCESS;
 }
 
 int raja_cdna4_server_destroy(raja_cdna4_server_s* sv)
 {
   if (!sv) return RAJA_CDNA4_SUCCESS;
+  RC_TRY(raja_cdna4_server_stop(sv));
   for (hipStream_t s : sv->streams) RC_HIP(hipStreamSynchronize(s));
   for (hipStream_t s : sv->streams) RC_HIP(hipStreamDestroy(s));
   for (Slot& sl : sv->slots)

@@ -818,6 +818,53 @@ int tune_sort(int key_dtype, void* keys, void* vals, int64_t n, int items, hipSt
   }
 }
 
+int64_t seg_tile() { return SortShape<uint32_t, uint32_t, true>::TILE; }
+
+size_t seg_desc_bytes(int64_t ntiles)
+{
+  return 2 * (size_t)ntiles * kRowStride * sizeof(uint32_t);
+}
+
+size_t seg_hist_bytes(int64_t nranges)
+{
+  return (size_t)nranges * 4 * kRadix * sizeof(uint32_t);
+}
+
+int seg_sort_table(int key_dtype, int val_bytes, int descending, const void* in_k, const void* in_v,
+                   void* out_k, void* out_v, void* alt_k, void* alt_v, const rs::SortTile* d_recs,
+                   int64_t ntiles, int64_t nranges, uint32_t* hist, uint32_t* desc, hipStream_t s)
+{
+  if (val_bytes != 0 && val_bytes != 4)
+  {
+    return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "segmented onesweep takes 4-byte values");
+  }
+#define RSORT_SEG_CASE(CODE, K)                                                                   \
+  case CODE:                                                                                      \
+    if (val_bytes == 4)                                                                           \
+      return descending                                                                           \
+                 ? seg_sort_core<K, uint32_t, true, true>(                                        \
+                       (const K*)in_k, (const uint32_t*)in_v, (K*)out_k, (uint32_t*)out_v,        \
+                       (K*)alt_k, (uint32_t*)alt_v, d_recs, ntiles, nranges, hist, desc, s)        \
+                 : seg_sort_core<K, uint32_t, true, false>(                                       \
+                       (const K*)in_k, (const uint32_t*)in_v, (K*)out_k, (uint32_t*)out_v,        \
+                       (K*)alt_k, (uint32_t*)alt_v, d_recs, ntiles, nranges, hist, desc, s);       \
+    return descending                                                                             \
+               ? seg_sort_core<K, uint32_t, false, true>((const K*)in_k, nullptr, (K*)out_k,      \
+                                                          nullptr, (K*)alt_k, nullptr, d_recs,    \
+                                                          ntiles, nranges, hist, desc, s)         \
+               : seg_sort_core<K, uint32_t, false, false>((const K*)in_k, nullptr, (K*)out_k,     \
+                                                           nullptr, (K*)alt_k, nullptr, d_recs,   \
+                                                           ntiles, nranges, hist, desc, s);
+  switch (key_dtype)
+  {
+    RSORT_SEG_CASE(RAJA_CDNA4_U32, uint32_t)
+    RSORT_SEG_CASE(RAJA_CDNA4_I32, int32_t)
+    RSORT_SEG_CASE(RAJA_CDNA4_F32, float)
+    default: return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "segmented onesweep takes 4-byte keys");
+  }
+#undef RSORT_SEG_CASE
+}
+
 int take_spin_timeouts(uint32_t* count)
 {
   uint32_t v = 0;

@@ -196,3 +196,47 @@ def test_segmented_scan_f64_matches_span_scans(raja):
         raja.exclusive_scan(x[a:b].clone(), one, raja.operators.plus, 0.25)
         assert np.array_equal(got[a:b].view(np.int64), one.cpu().numpy().view(np.int64)), s
     plan.close()
+
+
+@pytest.mark.parametrize("kdt,with_vals,desc", [(torch.int32, True, False), (torch.float32, False, True),
+                                                (torch.uint32, False, False), (torch.float32, True, False)])
+def test_segmented_sort_long_segments_one_pass_per_digit(raja, kdt, with_vals, desc):
+    """Segments longer than one block sort (8193 to 150000 keys, mixed with
+    short and empty ones) are sorted together by the segmented onesweep;
+    each must equal numpy's stable sort of its own keys (ascending, or the
+    stable descending order)."""
+    rng = np.random.default_rng(12)
+    lens = np.concatenate([rng.integers(8193, 150000, 40), [0, 1, 100, 8192, 8193, 16384, 16385],
+                           rng.integers(0, 8192, 60)])
+    rng.shuffle(lens)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(off[-1])
+    if kdt == torch.float32:
+        hk = rng.standard_normal(n).astype(np.float32)
+        hk[::7] = np.round(hk[::7], 1)  # duplicates
+        hk[::101] = -0.0  # equal to +0.0 under operator<: input order kept
+        hk[::103] = 0.0
+    elif kdt == torch.int32:
+        hk = rng.integers(-3000, 3000, n).astype(np.int32)
+    else:
+        hk = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    k = torch.from_numpy(hk.view(np.int32)).to(DEV).view(kdt) if kdt == torch.uint32 \
+        else torch.from_numpy(hk).to(DEV)
+    v = torch.arange(n, dtype=torch.int32, device=DEV) if with_vals else None
+    plan = raja.Segments(off)
+    print("segmented onesweep: %d keys in %d segments, %s, vals=%s, desc=%s"
+          % (n, len(lens), kdt, with_vals, desc), flush=True)
+    raja.segmented_sort_pairs(plan, k, v, raja.operators.greater if desc else raja.operators.less)
+    torch.cuda.synchronize()
+    gk = (k.view(torch.int32).cpu().numpy().view(np.uint32) if kdt == torch.uint32
+          else k.cpu().numpy())
+    gv = v.cpu().numpy() if with_vals else None
+    for s in range(len(lens)):
+        a, b = int(off[s]), int(off[s + 1])
+        seg = hk[a:b]
+        # stable order by key (descending: stable on the negated order)
+        idx = np.argsort(-seg.astype(np.float64) if desc else seg, kind="stable")
+        assert np.array_equal(gk[a:b], seg[idx]), s
+        if with_vals:
+            assert np.array_equal(gv[a:b], (a + idx).astype(np.int32)), s
+    plan.close()

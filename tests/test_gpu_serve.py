@@ -271,3 +271,77 @@ def test_serve_full_mix_at_config_sizes(raja, orc):
     raja.device_status()
     for a, b in zip(batched, single):
         assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
+
+
+def test_open_server_producer_threads(raja, orc):
+    """Requests enter while the server runs: two producer threads submit a
+    mixed stream in bursts (raja_cdna4_server_submit) while the main thread
+    polls completions; every request completes once, bursts form batched
+    units, and every result matches the oracle.  A stopped server refuses
+    submissions and starts again."""
+    import threading
+    import time
+
+    kinds, sizes = mix(raja, 400, seed=17, max_log2=17)
+    sizes[:4] = [9000, 30000, 65536, 0]
+    kinds[:4] = [raja.REQ_SORT_PAIRS_U32_I32] * 3 + [raja.REQ_SUM_F64]
+    pools = Pools(raja, kinds, sizes, seed=23)
+    reqs = pools.requests(raja)
+    torch.cuda.synchronize()  # inputs written on torch's stream are ready
+    srv = raja.Server(raja.SERVE_BATCHED, streams=4, max_inflight=16, max_batch=64,
+                      batch_limit=1 << 16, arena_bytes=1 << 30)
+    srv.start()
+    ticket_of = {}
+    lock = threading.Lock()
+    errors = []
+
+    def produce(part):
+        try:
+            for c in range(0, len(part), 25):
+                chunk = part[c:c + 25]
+                tk = srv.submit([reqs[i] for i in chunk])
+                with lock:
+                    for t, i in zip(tk, chunk):
+                        ticket_of[t] = i
+                time.sleep(0.002)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    idx = list(range(len(reqs)))
+    producers = [threading.Thread(target=produce, args=(idx[p::2],)) for p in range(2)]
+    print("open server: 2 producers, %d requests" % len(reqs), flush=True)
+    for t in producers:
+        t.start()
+    done = {}
+    deadline = time.time() + 120
+    while len(done) < len(reqs) and time.time() < deadline:
+        for tk, r in srv.poll(256, timeout_s=0.05):
+            assert tk not in done
+            done[tk] = (r.status, r.unit_size, r.start_s, r.done_s)
+    for t in producers:
+        t.join()
+    srv.stop()
+    assert not errors, errors
+    assert len(done) == len(reqs)
+    assert all(v[0] == 0 for v in done.values())
+    assert all(v[3] >= v[2] for v in done.values())
+    assert max(v[1] for v in done.values()) > 1  # waiting requests were batched
+    assert sorted(ticket_of) == sorted(done)
+    check_against_oracle(raja, orc, pools, pools.snapshot())
+
+    with pytest.raises(raja.RajaError):
+        srv.submit(reqs[0])
+    # started again, the same server serves a second stream
+    pools.reset_outputs()
+    torch.cuda.synchronize()
+    srv.start()
+    tks = srv.submit(reqs)
+    got = {}
+    deadline = time.time() + 120
+    while len(got) < len(tks) and time.time() < deadline:
+        for tk, r in srv.poll(512, timeout_s=0.05):
+            got[tk] = r.status
+    srv.stop()
+    srv.close()
+    assert sorted(got) == sorted(tks) and all(v == 0 for v in got.values())
+    check_against_oracle(raja, orc, pools, pools.snapshot())
