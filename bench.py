@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--no-per-segment", action="store_true",
                     help="skip the one-call-per-segment comparison (tens of thousands of launches; for counter runs)")
     ap.add_argument("--no-forall", action="store_true", help="skip the forall example programs (daxpy + dot + MinLoc, Sum/Min/Max/MinLoc)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="take the shapes the cache holds (defaults where absent) without measuring")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain 1 GiB device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -86,7 +88,7 @@ def main():
     n = 1 << args.log2n
 
     # ---- shapes: measured (or read from the on-disk cache) for the timed ops -------
-    shapes = tune_shapes(R, torch, n)
+    shapes = tune_shapes(R, torch, n, measure=not args.no_tune)
 
     # ---- headline: sort_pairs u32 uniform, iota values ------------------------------
     log("bench: sort_pairs n=2^%d setup" % args.log2n)
@@ -226,7 +228,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def tune_shapes(R, torch, n):
+def tune_shapes(R, torch, n, measure=True):
     """Shapes of the timed scans and the sort at length n: read from the
     shape cache, measured (raja_cdna4_autotune) where the cache has none."""
     out = {"cache": R.tune_cache_path()}
@@ -235,7 +237,7 @@ def tune_shapes(R, torch, n):
                          ("inclusive_scan_i64", R.TUNE_INCLUSIVE_SCAN, torch.int64)):
         choice, tuned = R.tune_query(op, dt, n)
         measured = False
-        if not tuned:
+        if not tuned and measure:
             try:
                 R.autotune(op, dt, n)
                 measured = True
@@ -461,9 +463,10 @@ def serving(R, torch, np, dev, nreq=4000):
         d = np.array([r.done_s for r in res]) - np.asarray(arrivals)
         return float(np.percentile(d, 50) * 1e3), float(np.percentile(d, 99) * 1e3)
 
-    # 16 streams and 64 units in flight measured best of the (streams, in flight,
-    # batch, limit) settings tried on this mix: 55k req/s against 42k at (4, 8).
-    cfg = dict(streams=16, max_inflight=64, max_batch=512, batch_limit=1 << 16, arena_bytes=64 << 30)
+    # Every request is batchable (batch_limit 2^24, the largest request):
+    # units of up to 512 waiting requests, one kernel (or one per digit pass)
+    # per kind over all of them.
+    cfg = dict(streams=16, max_inflight=64, max_batch=512, batch_limit=1 << 24, arena_bytes=64 << 30)
     out = {"workload": "%d requests: 40%% FP64 sum, 30%% int32 exclusive scan, 30%% sort_pairs "
                        "(uint32, int32); n = floor(2^(10+14u)); inputs resident; 64 GB arena" % nreq}
     servers = {
@@ -504,7 +507,7 @@ def serving(R, torch, np, dev, nreq=4000):
     lam = 0.6 * sat["batched"]
     arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / lam, nreq))
     reqs_p, _ = build(kinds, sizes, np.random.default_rng(9), arr)
-    for name, args_ in (("batched", (R.SERVE_BATCHED, 4, 8, 512, 1 << 16, 64 << 30, True)),
+    for name, args_ in (("batched", (R.SERVE_BATCHED, 4, 8, 512, 1 << 24, 64 << 30, True)),
                         ("per_request", (R.SERVE_PER_REQUEST, 1, 8, 1, 0, 64 << 30, True))):
         servers[name].close()
         srv = R.Server(*args_)

@@ -141,6 +141,9 @@ struct Tile
   int64_t end;
 };
 
+// Iterations per thread of one tile (a tile is kTileItems * B iterations).
+constexpr int kTileItems = 4;
+
 template<int B, typename Dispatch, typename Index, typename... Args>
 __global__ __launch_bounds__(B) void tiles_kernel(const Tile* __restrict__ tiles,
                                                   const LoopRecord* __restrict__ loops,
@@ -150,9 +153,24 @@ __global__ __launch_bounds__(B) void tiles_kernel(const Tile* __restrict__ tiles
   const Tile t = tiles[blockIdx.x];
   const LoopRecord L = loops[t.loop];
   const void* body = bytes + L.body_offset;
-  for (int64_t k = t.start + threadIdx.x; k < t.end; k += B)
+  if (t.end - t.start == (int64_t)kTileItems * B)
   {
-    Dispatcher<Dispatch, Index, Args...>::call(L, body, Index(L.begin + k), args...);
+    // A full tile: the thread's iterations in straight-line code, so the
+    // loads of later iterations can issue before earlier ones complete
+    // (a gather body is otherwise one dependent load chain per iteration).
+#pragma unroll
+    for (int u = 0; u < kTileItems; ++u)
+    {
+      Dispatcher<Dispatch, Index, Args...>::call(L, body, Index(L.begin + t.start + u * B + threadIdx.x),
+                                                 args...);
+    }
+  }
+  else
+  {
+    for (int64_t k = t.start + threadIdx.x; k < t.end; k += B)
+    {
+      Dispatcher<Dispatch, Index, Args...>::call(L, body, Index(L.begin + k), args...);
+    }
   }
 }
 
@@ -231,7 +249,7 @@ struct DeviceTables
     r.memcpy(d_loops, recs.data(), sizeof(LoopRecord) * recs.size());
     if (tiles)
     {
-      constexpr int64_t TILE = 4 * B;
+      constexpr int64_t TILE = (int64_t)kTileItems * B;
       std::vector<Tile> t;
       for (size_t l = 0; l < recs.size(); ++l)
       {

@@ -255,6 +255,22 @@ def exclusive_scan(in_, out, op=operators.plus, init=None):
     return out
 
 
+def exclusive_scan_noalloc(in_, out, op=operators.plus, init=None):
+    """exclusive_scan through raja_cdna4_exclusive_scan_noalloc: no device
+    allocation (a fixed per-GPU scratch array); calls on one GPU must not
+    overlap."""
+    if in_.numel() != out.numel() or in_.dtype != out.dtype:
+        raise RajaError("in and out spans must match in length and type")
+    _same_device(in_, out)
+    s = _prep(in_, out)
+    if init is None:
+        init = _IDENT[op](in_.dtype)
+    hold, p = _host_scalar(init, in_.dtype)
+    _check(lib().raja_cdna4_exclusive_scan_noalloc(_dt(in_), op, in_.data_ptr(), out.data_ptr(),
+                                                   in_.numel(), p, s))
+    return out
+
+
 def inclusive_scan_inplace(x, op=operators.plus):
     return inclusive_scan(x, x, op)
 

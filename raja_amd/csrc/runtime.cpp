@@ -257,6 +257,78 @@ TimedScope::~TimedScope()
   }
 }
 
+namespace
+{
+struct GateState
+{
+  std::recursive_mutex m;
+  hipEvent_t ev = nullptr;
+  hipStream_t last = nullptr;
+  bool recorded = false;
+};
+GateState g_gate[64];
+
+int gate_event(int dev, bool may_create, hipEvent_t* out)
+{
+  GateState& g = g_gate[dev];
+  if (!g.ev && may_create) RC_HIP(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
+  *out = g.ev;
+  return RAJA_CDNA4_SUCCESS;
+}
+}  // namespace
+
+int lookback_gate_setup()
+{
+  int dev = 0;
+  RC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return RAJA_CDNA4_SUCCESS;
+  std::lock_guard<std::recursive_mutex> lk(g_gate[dev].m);
+  hipEvent_t ev = nullptr;
+  return gate_event(dev, true, &ev);
+}
+
+LookbackGate::LookbackGate(hipStream_t s, bool may_create) : stream_(s)
+{
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipGetDevice(&device_) != hipSuccess || device_ < 0 || device_ >= 64 ||
+      hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+  {
+    (void)hipGetLastError();
+    return;
+  }
+  GateState& g = g_gate[device_];
+  g.m.lock();
+  hipEvent_t ev = nullptr;
+  status_ = gate_event(device_, may_create, &ev);
+  if (status_ != RAJA_CDNA4_SUCCESS || !ev)
+  {
+    g.m.unlock();
+    return;
+  }
+  active_ = true;
+  if (g.recorded && g.last != s)
+  {
+    const hipError_t e = hipStreamWaitEvent(s, ev, 0);
+    if (e != hipSuccess) status_ = hip_error(e, "hipStreamWaitEvent", __FILE__, __LINE__);
+  }
+}
+
+LookbackGate::~LookbackGate()
+{
+  if (!active_) return;
+  GateState& g = g_gate[device_];
+  if (hipEventRecord(g.ev, stream_) == hipSuccess)
+  {
+    g.recorded = true;
+    g.last = stream_;
+  }
+  else
+  {
+    (void)hipGetLastError();
+  }
+  g.m.unlock();
+}
+
 }  // namespace rc
 
 using namespace rc;
@@ -270,6 +342,7 @@ int raja_cdna4_init(int device)
   StreamCtx* c = nullptr;
   RC_TRY(get_ctx(nullptr, &c));
   RC_TRY(rscan::noalloc_setup(device));
+  RC_TRY(lookback_gate_setup());
   return RAJA_CDNA4_SUCCESS;
 }
 

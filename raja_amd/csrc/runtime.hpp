@@ -78,6 +78,37 @@ struct TimedScope
   ~TimedScope();
 };
 
+// One lookback launch at a time per device.  Blocks of a single-pass scan
+// or radix pass wait on lower tiles of the same launch; each XCD dispatches
+// its share of a launch in order, so one such launch always progresses.
+// Two running at once on different streams can deadlock: XCD x fills with
+// later blocks of launch A waiting on A's lowest unfinished block, which
+// XCD y cannot place because it is full of later blocks of launch B waiting
+// on B's lowest block, due on XCD x.  A LookbackGate serialises such work
+// across streams: it waits (on the device) for the device's previous gated
+// work when that was on another stream, and records an event after its
+// own.  Under stream capture it does nothing (a graph replays on one
+// stream).  `may_create` false: never creates the device's event (the
+// allocation-free entry point), so the gate is skipped until
+// raja_cdna4_init or a gated call created it.
+class LookbackGate
+{
+ public:
+  LookbackGate(hipStream_t s, bool may_create = true);
+  ~LookbackGate();
+  int status() const { return status_; }
+  LookbackGate(const LookbackGate&) = delete;
+  LookbackGate& operator=(const LookbackGate&) = delete;
+
+ private:
+  hipStream_t stream_;
+  int device_ = -1;
+  bool active_ = false;
+  int status_ = RAJA_CDNA4_SUCCESS;
+};
+// Creates the current device's gate event (raja_cdna4_init).
+int lookback_gate_setup();
+
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

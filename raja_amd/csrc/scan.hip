@@ -461,8 +461,11 @@ constexpr int kSpanChoices[3] = {0, sizeof(T) == 8 ? 6 : 8, sizeof(T) == 8 ? 3 :
 template<typename T, typename Op, bool EXCLUSIVE>
 int launch_scan_core(const T* in, T* out, int64_t n, T init, hipStream_t s, const char* name,
                      const rs::TileRec* tiles, int64_t ntiles, void* desc, uint32_t epoch_tag,
-                     bool clear, int mode = 1, int v = 0)
+                     bool clear, int mode = 1, int v = 0, bool gate_may_create = true)
 {
+  // one lookback launch at a time on the device (see rc::LookbackGate)
+  rc::LookbackGate gate(s, gate_may_create);
+  RC_TRY(gate.status());
   const bool aligned = (((uintptr_t)in | (uintptr_t)out) % 16) == 0;
   if (clear)
   {
@@ -811,7 +814,7 @@ int noalloc_scan_op(const T* in, T* out, int64_t n, T init, hipStream_t s, void*
   // memset, so the call can also be captured into a graph) zeroes them, and
   // the fixed tag 1 then marks this call's.  No timing events are created.
   return launch_scan_core<T, Op, true>(in, out, n, init, s, nullptr, nullptr, ntiles, desc,
-                                       1u << 2, true);
+                                       1u << 2, true, 1, 0, /*gate_may_create=*/false);
 }
 
 template<typename T>

@@ -538,8 +538,19 @@ struct Scheduler
       if (!sv->arena.alloc(bytes, &off)) break;  // wait for memory to come back
       const int slot = free_slots.back();
       free_slots.pop_back();
-      hipStream_t s = sv->streams[(size_t)(next_stream % nstreams)];
-      next_stream = (next_stream + 1) % nstreams;
+      // Units with lookback kernels (scans, radix passes) run one at a time on
+      // the device anyway (rc::LookbackGate): they all go to stream 0, so the
+      // gate never waits across streams; the others (sums, block sorts) go
+      // round-robin over the remaining streams.
+      const bool lookback =
+          batch ? (!sv->plan.scan_tiles.empty() || !sv->plan.sort_tiles.empty())
+                : (front.req.kind != RAJA_CDNA4_REQ_SUM_F64 && front.req.n > 1);
+      hipStream_t s = sv->streams[0];
+      if (!lookback && nstreams > 1)
+      {
+        s = sv->streams[(size_t)(1 + next_stream % (nstreams - 1))];
+        next_stream = (next_stream + 1) % (nstreams - 1);
+      }
       char* dev = sv->arena.base + off;
       Unit u;
       u.slot = slot;

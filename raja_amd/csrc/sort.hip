@@ -622,6 +622,8 @@ int seg_sort_core(const K* in_k, const V* in_v, K* out_k, V* out_v, K* alt_k, V*
   static_assert(P % 2 == 0, "an even pass count ends in the output");
   if (ntiles <= 0) return RAJA_CDNA4_SUCCESS;
   if (ntiles > 0x7FFFFFFF) return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "too many tiles");
+  rc::LookbackGate gate(s);  // one lookback launch at a time on the device
+  RC_TRY(gate.status());
   rc::TimedScope ts("segmented_onesweep", s);
   const int64_t words = nranges * P * kRadix;
   const int64_t cb = rc::ceil_div(words, 256);
@@ -693,6 +695,8 @@ int sort_core(const K* in_keys, const V* in_vals, K* keys, V* vals, int64_t n, h
       }
     }
   }
+  rc::LookbackGate gate(s);  // one lookback launch at a time on the device
+  RC_TRY(gate.status());
   // Out of place, an even pass count starts into the alternate buffers and
   // an odd one into the outputs, so the last pass always lands in the
   // outputs.  In place, passes start into the alternates and an odd count

@@ -103,9 +103,10 @@ def sort_cases():
         case["less"] = sorted(range(n), key=lambda j: keys[j])
         case["greater"] = sorted(range(n), key=lambda j: -keys[j])
         out.append(case)
-    # +-0.0 and duplicates in double keys: radix order puts -0.0 first
+    # +-0.0 and duplicates in double keys: -0.0 == +0.0 under operator<, so a
+    # stable sort keeps them in input order (Python's sorted compares the same)
     fk = [0.0, -0.0, 1.5, -2.25, 0.0, -0.0, 1.5, float("-inf"), float("inf"), -2.25]
-    order = sorted(range(len(fk)), key=lambda j: (fk[j], 0 if str(fk[j]).startswith("-0") else 1))
+    order = sorted(range(len(fk)), key=lambda j: fk[j])
     out.append({"n": len(fk), "keys": fk, "float": True, "less": order})
     return out
 

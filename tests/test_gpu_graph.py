@@ -67,3 +67,68 @@ def test_segmented_ops_graph_replays(raja):
         raja.exclusive_scan(xi, oi, raja.operators.plus, 0)
         torch.cuda.synchronize()
     plan.close()
+
+
+def test_graph_replays_reductions_and_allocation_free_scan(raja):
+    """A span reduction, a segmented (batched) reduction and the
+    allocation-free exclusive scan captured into one HIP graph on a side
+    stream and replayed four times, with eager work and synchronisations
+    between replays -- including an eager allocation-free scan of another
+    length that leaves its own descriptors in the shared scratch: every
+    replay gives the eager bits."""
+    rng = np.random.default_rng(33)
+    n = 3_000_017
+    lens = np.floor(2.0 ** (15 * rng.random(300))).astype(np.int64)
+    lens[:3] = [0, 1, 40000]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ns = int(off[-1])
+    print("graph replay (reductions, allocation-free scan): n=%d, %d segments" % (n, len(lens)),
+          flush=True)
+    x = torch.empty(max(n, ns), dtype=torch.float64, device=DEV)
+    raja.fill(raja.FILL_F64_NORMAL, x, seed=51)
+    xs = x[:ns]
+    xi = torch.empty(n, dtype=torch.int32, device=DEV)
+    raja.fill(raja.FILL_I32, xi, seed=52, a=-9, b=16)
+    plan = raja.Segments(off)
+    r = torch.empty(1, dtype=torch.float64, device=DEV)
+    sums = torch.empty(len(lens), dtype=torch.float64, device=DEV)
+    oi = torch.empty_like(xi)
+    other_in = torch.ones(70_001, dtype=torch.int32, device=DEV)
+    other_out = torch.empty_like(other_in)
+
+    def ops():
+        raja.reduce(x[:n], raja.operators.plus, 0.25, r)
+        raja.segmented_reduce(plan, xs, raja.operators.plus, 0.5, sums)
+        raja.exclusive_scan_noalloc(xi, oi, raja.operators.plus, 3)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):  # eager on the capture stream: scratch exists
+        ops()
+    torch.cuda.synchronize()
+    e_r, e_sums, e_oi = r.clone(), sums.clone(), oi.clone()
+    assert np.array_equal(e_oi.cpu().numpy(),
+                          np.concatenate([[3], 3 + np.cumsum(xi.cpu().numpy())[:-1]]).astype(np.int32))
+
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        ops()
+    torch.cuda.synchronize()
+    for rep in range(4):
+        r.fill_(-1.0)
+        sums.fill_(-1.0)
+        oi.fill_(-1)
+        torch.cuda.synchronize()
+        print("replay %d" % rep, flush=True)
+        g.replay()
+        torch.cuda.synchronize()
+        raja.device_status()
+        assert torch.equal(r.view(torch.int64), e_r.view(torch.int64)), rep
+        assert torch.equal(sums.view(torch.int64), e_sums.view(torch.int64)), rep
+        assert torch.equal(oi, e_oi), rep
+        # eager work between replays: the same scratch, another length
+        raja.exclusive_scan_noalloc(other_in, other_out, raja.operators.plus, 0)
+        raja.reduce_sum(x[:1000], 0.0)
+        torch.cuda.synchronize()
+        assert other_out[-1].item() == 70_000
+    plan.close()

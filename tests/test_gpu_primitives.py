@@ -283,7 +283,7 @@ def test_sort_golden(raja, golden):
             k = torch.tensor(case["keys"], dtype=torch.float64, device=DEV)
             v = torch.arange(n, dtype=torch.int32, device=DEV)
             raja.sort_pairs(k, v)
-            assert to_np(v).tolist() == case["less"]  # -0.0 before +0.0
+            assert to_np(v).tolist() == case["less"]  # -0.0 and +0.0 keep input order
             continue
         for dt in (torch.int32, torch.int64, torch.float64, torch.float32):
             for comp in ("less", "greater"):

@@ -247,18 +247,21 @@ def test_serve_full_mix_at_config_sizes(raja, orc):
     check_full_mix(raja, orc, pools, batched)
 
     # every request waiting at once: units of several requests form, and
-    # each result keeps its bits
-    print("serve full mix: saturation (all requests at t=0)", flush=True)
-    pools.reset_outputs()
-    srv = raja.Server(raja.SERVE_BATCHED, streams=16, max_inflight=64, max_batch=512,
-                      batch_limit=1 << 16, arena_bytes=64 << 30)
-    res, _ = srv.run(pools.requests(raja))
-    srv.close()
-    assert all(r.status == 0 for r in res)
-    assert max(r.unit_size for r in res) > 1
-    raja.device_status()
-    for a, b in zip(batched, pools.snapshot()):
-        assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
+    # each result keeps its bits -- with the small ones batched, and with
+    # every request batched (batch_limit 2^24: 2^24-element sums, scans and
+    # sorts inside batches)
+    for limit in (1 << 16, 1 << 24):
+        print("serve full mix: saturation (all requests at t=0), batch_limit %d" % limit, flush=True)
+        pools.reset_outputs()
+        srv = raja.Server(raja.SERVE_BATCHED, streams=16, max_inflight=64, max_batch=512,
+                          batch_limit=limit, arena_bytes=64 << 30)
+        res, _ = srv.run(pools.requests(raja))
+        srv.close()
+        assert all(r.status == 0 for r in res)
+        assert max(r.unit_size for r in res) > (40 if limit > 1 << 16 else 1)
+        raja.device_status()
+        for a, b in zip(batched, pools.snapshot()):
+            assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
 
     print("serve full mix: per-request mode", flush=True)
     pools.reset_outputs()

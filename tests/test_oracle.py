@@ -176,9 +176,9 @@ def test_sorts_golden(golden):
             k = np.array(case["keys"], np.float64)
             v = np.arange(case["n"], dtype=np.int32)
             ko, vo = O.seq_stable_sort_pairs(k, v)
-            # comparison sort keeps +-0.0 in input order; only the values of
-            # non-zero keys are pinned here (radix order is tested on the GPU)
+            # the comparison sort keeps +-0.0 (equal under operator<) in input order
             assert np.array_equal(ko, np.sort(k))
+            assert vo.tolist() == case["less"]
             continue
         n = case["n"]
         for dt in (np.int32, np.int64, np.float64):
