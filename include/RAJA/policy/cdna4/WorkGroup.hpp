@@ -16,6 +16,7 @@
 // body type.
 #pragma once
 
+#include <cstring>
 #include <vector>
 
 #include "RAJA/pattern/WorkGroup.hpp"
@@ -191,10 +192,15 @@ __global__ __launch_bounds__(B) void loop_kernel(const LoopRecord* __restrict__ 
 
 // Device copies of a group's loop bodies and records and, for unordered
 // runs, its tile table: built at the group's first run, kept for later
-// runs, released when the loop set changes or the group goes away.
+// runs, released when the loop set changes or the group goes away.  The
+// three parts are staged in one pinned host buffer and uploaded with one
+// asynchronous copy on the run's stream (no host wait: the staging buffer
+// lives until the tables are released, after the stream is synchronised).
 template<int B>
 struct DeviceTables
 {
+  unsigned char* d_all = nullptr;  // bodies, records, tiles in one device block
+  unsigned char* h_all = nullptr;  // pinned staging of the same bytes
   unsigned char* d_bytes = nullptr;
   LoopRecord* d_loops = nullptr;
   Tile* d_tiles = nullptr;
@@ -208,6 +214,8 @@ struct DeviceTables
   DeviceTables(DeviceTables&& o) noexcept { *this = std::move(o); }
   DeviceTables& operator=(DeviceTables&& o) noexcept
   {
+    std::swap(d_all, o.d_all);
+    std::swap(h_all, o.h_all);
     std::swap(d_bytes, o.d_bytes);
     std::swap(d_loops, o.d_loops);
     std::swap(d_tiles, o.d_tiles);
@@ -222,15 +230,16 @@ struct DeviceTables
   {
     if (!built) return;
     (void)hipStreamSynchronize(stream);
-    (void)raja_cdna4_device_free(d_bytes);
-    (void)raja_cdna4_device_free(d_loops);
-    if (d_tiles) (void)raja_cdna4_device_free(d_tiles);
-    d_bytes = nullptr;
+    (void)raja_cdna4_device_free(d_all);
+    (void)raja_cdna4_pinned_free(h_all);
+    d_all = h_all = d_bytes = nullptr;
     d_loops = nullptr;
     d_tiles = nullptr;
     ntiles = 0;
     built = false;
   }
+
+  static size_t round16(size_t v) { return (v + 15) / 16 * 16; }
 
   // Upload the loop bodies, records and (for unordered runs) the tile table.
   template<typename Storage>
@@ -242,15 +251,10 @@ struct DeviceTables
     {
       for (size_t l = 0; l < recs.size(); ++l) recs[l].device_fn = s.host_fns[l].device_fn();
     }
-    resources::detail::check_lib(raja_cdna4_device_malloc((void**)&d_bytes, s.used + 16), "wg bytes");
-    resources::detail::check_lib(
-        raja_cdna4_device_malloc((void**)&d_loops, sizeof(LoopRecord) * recs.size()), "wg loops");
-    r.memcpy(d_bytes, s.bytes.get(), s.used);
-    r.memcpy(d_loops, recs.data(), sizeof(LoopRecord) * recs.size());
+    std::vector<Tile> t;
     if (tiles)
     {
       constexpr int64_t TILE = (int64_t)kTileItems * B;
-      std::vector<Tile> t;
       for (size_t l = 0; l < recs.size(); ++l)
       {
         for (int64_t s0 = 0; s0 < recs[l].len; s0 += TILE)
@@ -258,12 +262,21 @@ struct DeviceTables
           t.push_back(Tile{(uint32_t)l, 0u, s0, std::min(recs[l].len, s0 + TILE)});
         }
       }
-      ntiles = (int64_t)t.size();
-      resources::detail::check_lib(raja_cdna4_device_malloc((void**)&d_tiles, sizeof(Tile) * t.size()),
-                                   "wg tiles");
-      r.memcpy(d_tiles, t.data(), sizeof(Tile) * t.size());
     }
-    r.wait();  // host staging vectors go out of scope
+    const size_t o_loops = round16(s.used + 16);
+    const size_t o_tiles = o_loops + round16(sizeof(LoopRecord) * recs.size());
+    const size_t total = o_tiles + sizeof(Tile) * t.size();
+    resources::detail::check_lib(raja_cdna4_device_malloc((void**)&d_all, total), "wg tables");
+    resources::detail::check_lib(raja_cdna4_pinned_malloc((void**)&h_all, total), "wg staging");
+    std::memcpy(h_all, s.bytes.get(), s.used);
+    std::memcpy(h_all + o_loops, recs.data(), sizeof(LoopRecord) * recs.size());
+    if (!t.empty()) std::memcpy(h_all + o_tiles, t.data(), sizeof(Tile) * t.size());
+    resources::detail::check_hip(hipMemcpyAsync(d_all, h_all, total, hipMemcpyHostToDevice, stream),
+                                 "wg tables upload");
+    d_bytes = d_all;
+    d_loops = reinterpret_cast<LoopRecord*>(d_all + o_loops);
+    d_tiles = t.empty() ? nullptr : reinterpret_cast<Tile*>(d_all + o_tiles);
+    ntiles = (int64_t)t.size();
     built = true;
   }
 };
@@ -300,8 +313,17 @@ struct WorkRunner<policy::cdna4::cdna4_work_t<B, A>, Order, Dispatch, Alloc, Ind
   per_run_storage run(WorkContainer const& s, resource_type r, Args... args) const
   {
     if (s.size() == 0) return 0;
-    if (!m_tables.built) m_tables.build(s, r, unordered, indirect);
     hipStream_t st = r.get_stream();
+    if (!m_tables.built)
+    {
+      m_tables.build(s, r, unordered, indirect);
+    }
+    else if (m_tables.stream != st)
+    {
+      // the tables were uploaded (and last read) on another stream
+      resources::detail::check_hip(hipStreamSynchronize(m_tables.stream), "wg stream switch");
+      m_tables.stream = st;
+    }
     if (unordered)
     {
       if (m_tables.ntiles > 0)
