@@ -229,17 +229,18 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   const int wave = wave_id();
   for (int i = tid; i < NW * kRadix; i += BLOCK) s_whist[i] = 0u;
   // Tile of this block.  Blocks go to the 8 XCDs round-robin (block b to
-  // XCD b % 8), and each XCD has its own L2.  Mapping 8 consecutive tiles of
-  // every group of 64 to one XCD lets that L2 merge the adjacent digit runs
+  // XCD b % 8), and each XCD has its own L2.  Mapping 32 consecutive tiles of
+  // every group of 256 to one XCD lets that L2 merge the adjacent digit runs
   // of neighbouring tiles (tile t's run of digit d ends where tile t+1's
-  // begins) before they reach HBM: one pass over 2^28 pairs took 0.997 ms
-  // against 1.276 ms with tile = block (0.976 ms with 32 per group of 256).
-  // The tiles of each XCD stay in increasing order, and a group is no larger
-  // than the 64 blocks one XCD holds, so the block of the lowest unfinished
-  // tile is always resident, whatever the XCD assignment; the last, partial
-  // group keeps tile = block.
+  // begins) before they reach HBM: one pass over 2^28 pairs took 1.276 ms
+  // with tile = block, 0.997 ms with 8 per group of 64 and 0.985 ms with 32
+  // per group of 256.  Each XCD dispatches its blocks in order and receives
+  // its tiles in increasing order, so every block before the lowest
+  // unfinished tile's block on that XCD is finished and that block is placed
+  // as soon as the XCD has room (lookback launches run one at a time per
+  // device, rc::LookbackGate); the last, partial group keeps tile = block.
   const int64_t tile = [] {
-    constexpr int64_t KX = 8, GX = 8 * KX;
+    constexpr int64_t KX = 32, GX = 8 * KX;
     const int64_t b = blockIdx.x, g = b / GX;
     if ((g + 1) * GX > (int64_t)gridDim.x) return b;
     const int64_t r = b - g * GX;

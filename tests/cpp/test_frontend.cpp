@@ -915,6 +915,88 @@ void test_cdna4_workgroups()
   dfree(dbuf);
 }
 
+// An unordered WorkGroup run captured into a HIP graph and replayed: the
+// group's device tables are built by an eager run on the capture stream,
+// so the capture holds the one tile kernel.  Each replay, after the inputs
+// change eagerly, must give what an eager run gives.
+struct ScaleBody
+{
+  const double* in;
+  double* out;
+  double a;
+  RAJA_HOST_DEVICE void operator()(RAJA::Index_type i) const { out[i] = a * in[i] + 1.0; }
+};
+
+void test_cdna4_workgroup_graph()
+{
+  const int nloops = 40;
+  std::vector<int> lens;
+  int total = 0;
+  for (int l = 0; l < nloops; ++l)
+  {
+    lens.push_back(l % 7 == 0 ? 70000 + 13 * l : 1 + 97 * l);
+    total += lens.back();
+  }
+  std::vector<double> hin(total);
+  for (int i = 0; i < total; ++i) hin[i] = 0.5 * i - 3.0;
+  double* din = to_device(hin);
+  std::vector<double> zero(total, 0.0);
+  double* dout = to_device(zero);
+  hipStream_t st = nullptr;
+  CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess, "graph: stream");
+  auto res = RAJA::resources::Cdna4::CdnaFromStream(st);
+  using Disp = RAJA::direct_dispatch<camp::list<RAJA::RangeSegment, ScaleBody>>;
+  using Pol = RAJA::WorkGroupPolicy<RAJA::cdna4_work_async<256>, RAJA::unordered_cdna4_loop_tiles,
+                                    RAJA::constant_stride_array_of_objects, Disp>;
+  using Pool = RAJA::WorkPool<Pol, RAJA::Index_type, RAJA::xargs<>, std::allocator<char>>;
+  Pool pool;
+  int off = 0;
+  for (int l = 0; l < nloops; ++l)
+  {
+    pool.enqueue(RAJA::RangeSegment(0, lens[l]), ScaleBody{din + off, dout + off, 1.0 + l});
+    off += lens[l];
+  }
+  auto group = pool.instantiate();
+  group.run(res);  // eager: builds and uploads the device tables
+  res.wait();
+  auto expect = [&](const std::vector<double>& in) {
+    std::vector<double> r(total);
+    int o = 0;
+    for (int l = 0; l < nloops; ++l)
+    {
+      for (int i = 0; i < lens[l]; ++i) r[o + i] = (1.0 + l) * in[o + i] + 1.0;
+      o += lens[l];
+    }
+    return r;
+  };
+  CHECK(to_host(dout, total) == expect(hin), "cdna4 workgroup eager run before capture");
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  group.run(res);
+  ok = ok && hipStreamEndCapture(st, &graph) == hipSuccess;
+  ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  CHECK(ok, "cdna4 workgroup run captured into a graph");
+  bool all = ok;
+  for (int rep = 0; ok && rep < 4; ++rep)
+  {
+    for (int i = 0; i < total; ++i) hin[i] = (double)((i * (rep + 3)) % 1001) - 500.0;
+    CHECK(hipMemcpy(din, hin.data(), sizeof(double) * total, hipMemcpyHostToDevice) == hipSuccess, "graph: input");
+    CHECK(hipMemset(dout, 0, sizeof(double) * total) == hipSuccess, "graph: clear");
+    all = all && hipGraphLaunch(exec, st) == hipSuccess;
+    all = all && hipStreamSynchronize(st) == hipSuccess;
+    all = all && to_host(dout, total) == expect(hin);
+  }
+  CHECK(all, "cdna4 workgroup graph replays (4, inputs changed eagerly between them)");
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  group.clear();
+  dfree(din);
+  dfree(dout);
+  (void)raja_cdna4_stream_release(st);
+  (void)hipStreamDestroy(st);
+}
+
 void test_cdna4_launch()
 {
   LaunchData L;
@@ -1212,6 +1294,7 @@ int main()
   test_cdna4_segments_resources();
   test_cdna4_scans_sorts();
   test_cdna4_workgroups();
+  test_cdna4_workgroup_graph();
   test_cdna4_launch();
   test_plugins<RAJA::cdna4_exec<256>>("cdna4", RAJA::Platform::cdna4);
   test_cdna4_multi();
