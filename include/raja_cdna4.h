@@ -322,13 +322,17 @@ typedef struct
   int streams;         /* streams for launch units, 1..16 (per-request mode uses 1)  */
   int max_inflight;    /* launch units in flight, 1..64                              */
   int max_batch;       /* requests per batch, >= 1                                   */
-  int64_t batch_limit; /* sum/scan requests with n <= batch_limit join batches;
-                          sorts join when n <= 8192; at most 2^27 (the batched
-                          sum folds at most 4096 tiles of 32768 doubles)             */
+  int64_t batch_limit; /* requests with n <= batch_limit join batches (sorts of up
+                          to 8192 pairs always); at most 2^27 (the batched sum
+                          folds at most 4096 tiles of 32768 doubles)                 */
   int64_t arena_bytes; /* device working memory of the server                       */
   int honor_arrivals;  /* 1: a request starts no earlier than its arrival_s;
                           0: every request has arrived at time 0 (saturation)        */
   int pad;
+  int64_t batch_bytes; /* a batch stops growing once the algorithmic bytes of its
+                          requests (8n sum, 8n scan, 68n sort) reach this; 0: no
+                          cap.  Caps how long one unit runs, so that requests
+                          arriving behind it wait less                             */
 } raja_cdna4_serve_config;
 
 typedef struct

@@ -65,7 +65,8 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("streams", ctypes.c_int),
                 ("max_inflight", ctypes.c_int), ("max_batch", ctypes.c_int),
                 ("batch_limit", ctypes.c_int64), ("arena_bytes", ctypes.c_int64),
-                ("honor_arrivals", ctypes.c_int), ("pad", ctypes.c_int)]
+                ("honor_arrivals", ctypes.c_int), ("pad", ctypes.c_int),
+                ("batch_bytes", ctypes.c_int64)]
 
 
 class Request(ctypes.Structure):
@@ -419,9 +420,9 @@ class Server:
     small requests, large ones alone on round-robin streams."""
 
     def __init__(self, mode=SERVE_BATCHED, streams=4, max_inflight=8, max_batch=512,
-                 batch_limit=1 << 16, arena_bytes=1 << 30, honor_arrivals=False):
+                 batch_limit=1 << 16, arena_bytes=1 << 30, honor_arrivals=False, batch_bytes=0):
         self.config = ServeConfig(mode, streams, max_inflight, max_batch, batch_limit,
-                                  arena_bytes, 1 if honor_arrivals else 0, 0)
+                                  arena_bytes, 1 if honor_arrivals else 0, 0, int(batch_bytes))
         _prep()
         h = ctypes.c_void_p()
         _check(lib().raja_cdna4_server_create(ctypes.byref(self.config), ctypes.byref(h)))

@@ -135,6 +135,13 @@ size_t single_ws(const raja_cdna4_request& r)
   }
 }
 
+// Algorithmic bytes of a request: 8 per element for a sum or a scan, 68 per
+// pair for a sort (four passes of 16 and the histogram read).
+int64_t request_bytes(const raja_cdna4_request& r)
+{
+  return r.kind == RAJA_CDNA4_REQ_SORT_PAIRS_U32_I32 ? 68 * r.n : 8 * r.n;
+}
+
 // Upper bound of the bytes one batchable request adds to a batch's device
 // image (records, partials, descriptors).
 size_t batch_bytes_bound(const raja_cdna4_request& r)
@@ -514,14 +521,17 @@ struct Scheduler
       else
       {
         size_t bound = kBatchFixedBound;
+        int64_t work = 0;
         reqs.clear();
         for (size_t j = 0; j < ready.size() && (int64_t)reqs.size() < cfg.max_batch &&
                            is_batchable(cfg, ready[j].req);
              ++j)
         {
+          if (cfg.batch_bytes > 0 && work >= cfg.batch_bytes) break;
           bound += batch_bytes_bound(ready[j].req);
           if (bound > sv->arena.size && !reqs.empty()) break;
           reqs.push_back(&ready[j].req);
+          work += request_bytes(ready[j].req);
         }
         take = reqs.size();
         sv->plan.build(reqs);
@@ -678,7 +688,7 @@ int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_s
   const raja_cdna4_serve_config& c = *config;
   if ((c.mode != RAJA_CDNA4_SERVE_BATCHED && c.mode != RAJA_CDNA4_SERVE_PER_REQUEST) ||
       c.streams < 1 || c.streams > 16 || c.max_inflight < 1 || c.max_inflight > 64 ||
-      c.max_batch < 1 || c.batch_limit < 0 || c.arena_bytes < (int64_t)kAlign)
+      c.max_batch < 1 || c.batch_limit < 0 || c.arena_bytes < (int64_t)kAlign || c.batch_bytes < 0)
   {
     return rc::set_error(RAJA_CDNA4_ERROR_INVALID_ARGUMENT, "bad server configuration");
   }

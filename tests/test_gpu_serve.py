@@ -139,6 +139,18 @@ def test_serve_mix_matches_oracle_and_modes_agree(raja, orc):
     for a, b in zip(batched, single):
         assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
 
+    # batches capped at 4 MB of algorithmic bytes: more, smaller units, same bits
+    print("serve mix: batched with batch_bytes 4 MB", flush=True)
+    pools.reset_outputs()
+    srv = raja.Server(raja.SERVE_BATCHED, streams=4, max_inflight=8, max_batch=64,
+                      batch_limit=1 << 18, arena_bytes=1 << 30, batch_bytes=4 << 20)
+    res, _ = srv.run(pools.requests(raja))
+    srv.close()
+    assert all(r.status == 0 for r in res)
+    assert max(r.unit_size for r in res) > 1  # some batches formed under the cap
+    for a, b in zip(pools.snapshot(), single):
+        assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
+
     # a sum served in a batch has the bits of the span reduction of its range
     for i, (k, (o, n)) in enumerate(zip(kinds, pools.spans)):
         if k == raja.REQ_SUM_F64 and i % 5 == 0:
