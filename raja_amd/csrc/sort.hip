@@ -11,7 +11,8 @@
 //   2. the host reads the histograms back for large inputs and skips every
 //      pass whose digit is the same for all keys;
 //   3. each pass is one kernel: each block owns one tile (16384 keys of 32
-//      bits; groups of 8 consecutive tiles go to one XCD), loads its keys, ranks them inside each wave by 8 ballots (peer lanes
+//      bits; 32 consecutive tiles of every 256 go to one XCD), loads its keys,
+//      ranks them inside each wave by 8 ballots (peer lanes
 //      with the same digit) against per-wave LDS digit counters, publishes
 //      its per-digit counts, stages the keys in LDS in digit order, loads
 //      its values, looks back over predecessor tiles for each digit (one
@@ -38,7 +39,12 @@ constexpr uint32_t kCountMask = 0x3FFFFFFFu;
 // A descriptor row: 256 digit words, padded to 264 words so rows of
 // neighbouring tiles start in different cache-line sets.
 constexpr int kRowStride = kRadix + 8;
+// Predecessor rows a look-back step reads with all loads in flight.
+constexpr int kLookbackRows = 8;
 constexpr int64_t kMaxSortN = (int64_t)kCountMask;  // counts live in 30 bits
+// Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
+// the other fields zero (stride 0: offsets and the size are in bytes).
+constexpr int kBufferWord3 = 0x00020000;
 
 // Lookback waits that outlived SpinBound's limit (see raja_cdna4_device_status).
 static __device__ uint32_t g_spin_timeouts;
@@ -297,37 +303,52 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
 
   // Wave-local ranks: rank = digit count so far in this wave + lower peers.
   uint32_t rank[ITEMS];
-  const uint64_t lt_mask = lanemask_lt();
+  auto rank_items = [&](auto full_c) {
+    constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j)
-  {
-    const bool valid = j * wave_size + lane < wvalid;
-    const uint32_t d = digit_of<K, DESC>(k[j], shift);
-    // Peers: lanes whose digit equals this lane's, bit by bit.  sx is 0 or
-    // all ones (the bit sign-extended); a lane's bit differs from lane m's
-    // where (ballot ^ sx) has bit m set, so acc ORs those in and the peers
-    // are the valid lanes outside acc.  v_bitop3 does acc | (ballot ^ sx) in
-    // one instruction per half.
-    const uint64_t vb = ballot(valid);
-    uint32_t accl = 0u, acch = 0u;
-#pragma unroll
-    for (int b = 0; b < 8; ++b)
+    for (int j = 0; j < ITEMS; ++j)
     {
-      int32_t sx = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
-      asm volatile("" : "+v"(sx));  // keep the one-instruction extract
-      const uint64_t bal = ballot(sx != 0);
-      accl = __builtin_amdgcn_bitop3_b32(accl, (uint32_t)bal, (uint32_t)sx, 0xF6);
-      acch = __builtin_amdgcn_bitop3_b32(acch, (uint32_t)(bal >> 32), (uint32_t)sx, 0xF6);
+      const bool valid = FULL || j * wave_size + lane < wvalid;
+      const uint32_t d = digit_of<K, DESC>(k[j], shift);
+      // Peers: the valid lanes whose digit equals this lane's, bit by bit.
+      // sx is 0 or all ones (the bit sign-extended); lane m agrees with this
+      // lane on the bit where ~(ballot ^ sx) has bit m set, so
+      // p = p & ~(ballot ^ sx), one v_bitop3 per half, keeps the lanes that
+      // agree on every bit.  A full tile starts from all lanes, a partial
+      // one from the valid lanes.
+      uint32_t pl = ~0u, ph = ~0u;
+      if constexpr (!FULL)
+      {
+        const uint64_t vb = ballot(valid);
+        pl = (uint32_t)vb;
+        ph = (uint32_t)(vb >> 32);
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+      {
+        int32_t sx = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
+        asm volatile("" : "+v"(sx));  // keep the one-instruction extract
+        const uint64_t bal = ballot(sx != 0);
+        pl = __builtin_amdgcn_bitop3_b32(pl, (uint32_t)bal, (uint32_t)sx, 0x90);
+        ph = __builtin_amdgcn_bitop3_b32(ph, (uint32_t)(bal >> 32), (uint32_t)sx, 0x90);
+      }
+      // Peers in lower lanes: v_mbcnt counts the mask's bits below this lane.
+      const uint32_t lower = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
+      uint32_t cnt = 0;
+      if (valid) cnt = s_whist[wave * kRadix + d];
+      rank[j] = (cnt + lower) | (d << 16);  // digit kept beside the rank
+      asm volatile("" : "+v"(rank[j]));  // one register per item from here on
+      if (valid && lower == 0)
+      {
+        s_whist[wave * kRadix + d] = cnt + (uint32_t)__builtin_popcount(pl) + (uint32_t)__builtin_popcount(ph);
+      }
+      __builtin_amdgcn_sched_barrier(0);  // rank one item at a time: few live masks
     }
-    const uint64_t peers = vb & ~(((uint64_t)acch << 32) | accl);
-    const uint32_t lower = (uint32_t)__popcll(peers & lt_mask);
-    uint32_t cnt = 0;
-    if (valid) cnt = s_whist[wave * kRadix + d];
-    rank[j] = (cnt + lower) | (d << 16);  // digit kept beside the rank
-    asm volatile("" : "+v"(rank[j]));  // one register per item from here on
-    if (valid && lower == 0) s_whist[wave * kRadix + d] = cnt + (uint32_t)__popcll(peers);
-    __builtin_amdgcn_sched_barrier(0);  // rank one item at a time: few live masks
-  }
+  };
+  if (full)
+    rank_items(std::true_type{});
+  else
+    rank_items(std::false_type{});
   __syncthreads();
 
   // Per digit (thread = digit): wave offsets, tile count, publication and
@@ -406,23 +427,23 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   {
     if (tid < kRadix)
     {
-      // Descriptors tile-1, tile-2, ... are read R at a time with all loads
-      // in flight; the fold stops at the first inclusive count.  A
-      // descriptor not yet published is re-read.
+      // Descriptors tile-1, tile-2, ... are read kLookbackRows at a time
+      // with all loads in flight; the fold stops at the first inclusive
+      // count.  A descriptor not yet published is re-read.
       const int d = tid;
       bool done = false;
       SpinBound bound;
-      for (int64_t t0 = tile - 1; !done; t0 -= 8)
+      for (int64_t t0 = tile - 1; !done; t0 -= kLookbackRows)
       {
-        uint32_t w[8];
+        uint32_t w[kLookbackRows];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kLookbackRows; ++u)
         {
           const int64_t t = t0 - u;
           w[u] = (t >= tfirst) ? atomic_load_agent(&desc_cur[t * kRowStride + d]) : (kInc << 30);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kLookbackRows; ++u)
         {
           if (done) break;
           const int64_t t = t0 - u;
@@ -461,11 +482,18 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
   // (positions past tile_n read stale LDS inside the buffer and are not
   // written), then the offsets, then the stores.  Full tiles skip the
   // position test.  Destinations are < n < 2^30 for consistent histograms;
-  // the unsigned test keeps an inconsistent one from writing out of range.
+  // an inconsistent one must not write out of range: 4-byte elements are
+  // stored through a buffer resource of n elements, whose range check drops
+  // a store at or past the end (no compare or branch per store), wider ones
+  // test the index.
   constexpr int G = 8;
   static_assert(ITEMS % G == 0, "scatter groups");
   const uint32_t n32 = (uint32_t)n;
   const bool full_tile = tile_n == TILE;
+  const __amdgpu_buffer_rsrc_t keys_rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(keys_out, (short)0, (int)(n32 * 4u), kBufferWord3);
+  const __amdgpu_buffer_rsrc_t vals_rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(vals_out, (short)0, (int)(n32 * 4u), kBufferWord3);
   auto scatter_keys = [&](auto full_c) {
     constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
@@ -488,7 +516,17 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
         const int m = m0 + g;
         const int i = tid + m * BLOCK;
         const uint32_t d = (uint32_t)(off[g] + i);
-        if ((FULL || i < tile_n) && d < n32) keys_out[d] = key[g];
+        if constexpr (sizeof(K) == 4)
+        {
+          if (FULL || i < tile_n)
+          {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, key[g]), keys_rsrc, (int)(d << 2), 0, 0);
+          }
+        }
+        else if ((FULL || i < tile_n) && d < n32)
+        {
+          keys_out[d] = key[g];
+        }
         if (HAS_VALS) dpk[m >> 2] |= dg[g] << (8 * (m & 3));
       }
       __builtin_amdgcn_sched_barrier(0);  // one group in flight: bounded registers
@@ -526,7 +564,17 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
         {
           const int i = tid + (m0 + g) * BLOCK;
           const uint32_t d = (uint32_t)(off[g] + i);
-          if ((FULL || i < tile_n) && d < n32) vals_out[d] = val[g];
+          if constexpr (sizeof(V) == 4)
+          {
+            if (FULL || i < tile_n)
+            {
+              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val[g]), vals_rsrc, (int)(d << 2), 0, 0);
+            }
+          }
+          else if ((FULL || i < tile_n) && d < n32)
+          {
+            vals_out[d] = val[g];
+          }
         }
         __builtin_amdgcn_sched_barrier(0);
       }
