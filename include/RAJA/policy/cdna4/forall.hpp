@@ -85,14 +85,19 @@ namespace cdna4
 namespace detail
 {
 
-// Launch context for reducer registration: while it is set, a host-side copy
-// of a reducer is the copy that travels to the kernel and gets block slots.
+// Launch context for reducer registration: while `setup` is set, a host-side
+// copy of a reducer is the copy that travels to the kernel and gets block
+// slots; while `probe` is set, a copy only counts itself in `probed` (how a
+// launch learns whether a loop body carries reducers before choosing its
+// grid).
 struct LaunchInfo
 {
   bool setup = false;
   int nblocks = 0;
   hipStream_t stream = nullptr;
   int block_threads = 0;  // threads per block of the launch
+  bool probe = false;
+  int probed = 0;
 };
 
 inline LaunchInfo& launch_info()
@@ -211,7 +216,11 @@ public:
     m_device_private = true;
 #else
     detail::LaunchInfo& li = detail::launch_info();
-    if (li.setup)
+    if (li.probe)
+    {
+      ++li.probed;
+    }
+    else if (li.setup)
     {
       m_slots = detail::acquire_slots<T>(li.nblocks);
       std::lock_guard<std::mutex> g(m_state->lock);
@@ -303,24 +312,34 @@ namespace detail
 
 // ---- the forall kernel -------------------------------------------------------
 
-constexpr int kForallItems = 6;  // indices per thread per tile
+// Launch shapes (indices per thread per tile, threads per CU of the default
+// grid).  A loop whose body carries reducers (expt::Reduce parameters or
+// captured old-style reducers) takes 8 indices per thread at 512 threads per
+// CU; any other loop 6 at 1024.  The grid then depends only on the range,
+// the block size, the CU count and whether the loop reduces -- never on the
+// body's register use -- so both reducer styles over the same loop give the
+// same bits.  Measured with B = 256 (indices per thread / threads per CU,
+// one MI355X): one forall with four reducers over 2^28 FP64 0.399 ms at
+// 4/2048, 0.376 at 6/1024, 0.359 at 8/512, 0.366 at 12/512, 0.40 at
+// 16/512; dot + MinLoc over 2^28 0.715 and 0.696 ms at 6/1024 and 8/512;
+// daxpy over 2^28 1.206 and 1.362 ms, over 2^24 0.071 and 0.103 ms (two
+// consecutive indices per lane, so a read-only body's loads merge into
+// 16-byte ones, did not beat 8/512 for reducing loops: 0.361 ms).
+constexpr int kForallItems = 6;
+constexpr int kForallThreadsPerCU = 1024;
+constexpr int kReduceItems = 8;
+constexpr int kReduceThreadsPerCU = 512;
 
-// Blocks of a forall grid: 1024 threads per CU whatever the kernel's
-// occupancy, so the grid (and the reducers' fold) does not depend on the body.
-// Measured with B = 256 (indices per thread / threads per CU): one forall
-// with four reducers over 2^28 FP64 0.399 ms at 4/2048, 0.367 at 6/1024,
-// 0.347 at 8/512; daxpy over 2^24 0.074, 0.071 and 0.103 ms.
-inline Index_type forall_grid_cap(int block)
+inline Index_type forall_grid_cap(int block, int threads_per_cu)
 {
-  return (Index_type)raja_cdna4_num_cus() * (Index_type)std::max(1, 1024 / block);
+  return (Index_type)raja_cdna4_num_cus() * (Index_type)std::max(1, threads_per_cu / block);
 }
 
-template<typename EXEC_POL, int BLOCK, typename Seg, typename Body, typename ForallParam>
+template<typename EXEC_POL, int BLOCK, int U, typename Seg, typename Body, typename ForallParam>
 __global__ __launch_bounds__(BLOCK) void forall_kernel(Seg seg, Body body, ForallParam f_params,
                                                        Index_type len)
 {
   Body b(body);  // device-private copy: its reducers fold per block at scope end
-  constexpr int U = kForallItems;
   const Index_type tstride = (Index_type)gridDim.x * BLOCK * U;
   for (Index_type base = (Index_type)blockIdx.x * BLOCK * U; base < len; base += tstride)
   {
@@ -452,21 +471,18 @@ namespace policy
 namespace cdna4
 {
 
-template<int B, bool Async, int G, typename Iterable, typename Body, typename ForallParam>
-std::enable_if_t<expt::type_traits::is_ForallParamPack<ForallParam>::value,
-                 resources::EventProxy<resources::Cdna4>>
-forall_impl(resources::Cdna4 res, const cdna4_exec_t<B, Async, G>& pol, Iterable&& it, Body&& body,
-            ForallParam f_params)
+// One forall launch of shape (U indices per thread, TPC threads per CU).
+template<int U, int TPC, int B, bool Async, int G, typename Seg, typename BodyT, typename Body,
+         typename ForallParam>
+void launch_forall(resources::Cdna4& res, const cdna4_exec_t<B, Async, G>& pol, const Seg& seg,
+                   Body&& body, ForallParam& f_params, Index_type len)
 {
   using EXEC_POL = cdna4_exec_t<B, Async, G>;
-  using Seg = std::decay_t<Iterable>;
-  using BodyT = std::decay_t<Body>;
   constexpr bool has_params = !expt::type_traits::is_ForallParamPack_empty<ForallParam>::value;
-  const Index_type len = (Index_type)it.size();
-  auto kernel = RAJA::cdna4::detail::forall_kernel<EXEC_POL, B, Seg, BodyT, ForallParam>;
-  constexpr Index_type per_block = (Index_type)B * RAJA::cdna4::detail::kForallItems;
+  auto kernel = RAJA::cdna4::detail::forall_kernel<EXEC_POL, B, U, Seg, BodyT, ForallParam>;
+  constexpr Index_type per_block = (Index_type)B * U;
   Index_type grid = (len + per_block - 1) / per_block;
-  const Index_type cap = (G > 0) ? G : RAJA::cdna4::detail::forall_grid_cap(B);
+  const Index_type cap = (G > 0) ? G : RAJA::cdna4::detail::forall_grid_cap(B, TPC);
   if (grid > cap) grid = cap;
   RAJA::cdna4::detail::cdna4Info info(res, dim3((unsigned)grid), dim3((unsigned)B));
   if constexpr (has_params) expt::ParamMultiplexer::parampack_init(pol, f_params, info);
@@ -475,7 +491,6 @@ forall_impl(resources::Cdna4 res, const cdna4_exec_t<B, Async, G>& pol, Iterable
   li = RAJA::cdna4::detail::LaunchInfo{true, (int)grid, stream, B};
   BodyT launch_body(body);  // old-style reducers in the body take their block slots here
   li.setup = false;
-  Seg seg(it);
   hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(B), 0, stream, seg, launch_body, f_params,
                      len);
   resources::detail::check_hip(hipGetLastError(), "forall launch");
@@ -486,6 +501,48 @@ forall_impl(resources::Cdna4 res, const cdna4_exec_t<B, Async, G>& pol, Iterable
   else if (!Async)
   {
     res.wait();
+  }
+}
+
+template<int B, bool Async, int G, typename Iterable, typename Body, typename ForallParam>
+std::enable_if_t<expt::type_traits::is_ForallParamPack<ForallParam>::value,
+                 resources::EventProxy<resources::Cdna4>>
+forall_impl(resources::Cdna4 res, const cdna4_exec_t<B, Async, G>& pol, Iterable&& it, Body&& body,
+            ForallParam f_params)
+{
+  using Seg = std::decay_t<Iterable>;
+  using BodyT = std::decay_t<Body>;
+  namespace dd = RAJA::cdna4::detail;
+  constexpr bool has_params = !expt::type_traits::is_ForallParamPack_empty<ForallParam>::value;
+  const Index_type len = (Index_type)it.size();
+  Seg seg(it);
+  if constexpr (has_params)
+  {
+    launch_forall<dd::kReduceItems, dd::kReduceThreadsPerCU, B, Async, G, Seg, BodyT>(
+        res, pol, seg, body, f_params, len);
+  }
+  else
+  {
+    // Does the body carry old-style reducers?  A probe copy counts them.
+    dd::LaunchInfo& li = dd::launch_info();
+    li = dd::LaunchInfo{};
+    li.probe = true;
+    {
+      BodyT probe(body);
+      (void)probe;
+    }
+    const bool reducing = li.probed > 0;
+    li = dd::LaunchInfo{};
+    if (reducing)
+    {
+      launch_forall<dd::kReduceItems, dd::kReduceThreadsPerCU, B, Async, G, Seg, BodyT>(
+          res, pol, seg, body, f_params, len);
+    }
+    else
+    {
+      launch_forall<dd::kForallItems, dd::kForallThreadsPerCU, B, Async, G, Seg, BodyT>(
+          res, pol, seg, body, f_params, len);
+    }
   }
   return resources::EventProxy<resources::Cdna4>(res);
 }

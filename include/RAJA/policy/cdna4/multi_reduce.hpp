@@ -174,7 +174,11 @@ public:
     (void)s_fold_kernel;
 #else
     RAJA::cdna4::detail::LaunchInfo& li = RAJA::cdna4::detail::launch_info();
-    if (li.setup && m_nbins > 0)
+    if (li.probe)
+    {
+      ++li.probed;
+    }
+    else if (li.setup && m_nbins > 0)
     {
       const size_t row_bytes = m_nbins * sizeof(T);
       size_t nrows = (size_t)li.nblocks;

@@ -1122,14 +1122,17 @@ void test_cdna4_multi()
 // Floating-point multi-reductions are deterministic: each wave folds its
 // calls into a private row (same-bin lanes in lane order, calls in program
 // order) and get() folds the rows in wave order.  The host below replays
-// that order for the forall's launch shape (tiles of 6*B indices, a grid of
-// min(tiles, CUs * 1024 / B) blocks), so the device bits must equal it, for
+// that order for the launch shape of a loop that reduces (tiles of
+// kReduceItems*B indices, a grid of min(tiles, CUs * kReduceThreadsPerCU / B)
+// blocks: 8*B and 512 threads per CU), so the device bits must equal it, for
 // two block sizes, and repeats must give the same bits.
 template<int B>
 std::vector<double> emulate_wave_rows(const std::vector<double>& x, int nb, size_t (*bin)(long long))
 {
-  const long long N = (long long)x.size(), U = 6, per_block = (long long)B * U;
-  const long long cap = (long long)raja_cdna4_num_cus() * std::max(1, 1024 / B);
+  const long long N = (long long)x.size(), U = RAJA::cdna4::detail::kReduceItems;
+  const long long per_block = (long long)B * U;
+  const long long cap =
+      (long long)raja_cdna4_num_cus() * std::max(1, RAJA::cdna4::detail::kReduceThreadsPerCU / B);
   const long long grid = std::min((N + per_block - 1) / per_block, cap);
   const int wpb = B / 64;
   std::vector<double> rows((size_t)(grid * wpb) * nb, 0.0);
