@@ -23,7 +23,26 @@ using namespace RAJA::cdna4::dev;
 constexpr int kFinalWaves = 16;
 constexpr int kFinalMaxGroups = 1024;  // spans up to 2^26 tiles
 
-// The two kernels over caller-provided partials (ntiles values).
+// Whether the first tree level runs as its own kernel (rt::group_kernel):
+// for accumulators wider than 8 bytes once the final block would fold more
+// than one group per wave (the fused Sum/Min/Max/MinLoc over 2^28 FP64:
+// 8192 tile values, 20.5 us in one block against 10.5 us with the first
+// level spread over 128 blocks, launches included; same bits).
+template<typename Op>
+bool split_first_level(int64_t ntiles)
+{
+  return sizeof(typename Op::acc_t) > 8 && rc::ceil_div(ntiles, (int64_t)64) > kFinalWaves;
+}
+
+// Tile values the partials buffer holds: ntiles, plus the group values
+// when the first level is split.
+template<typename Op>
+int64_t partial_slots(int64_t ntiles)
+{
+  return ntiles + (split_first_level<Op>(ntiles) ? rc::ceil_div(ntiles, (int64_t)64) : 0);
+}
+
+// The kernels over caller-provided partials (partial_slots values).
 template<typename T, typename Op>
 int launch_reduce_core(const T* x, int64_t n, typename Op::acc_t init, typename Op::acc_t* out,
                        typename Op::acc_t* partials, hipStream_t s, const char* name)
@@ -38,9 +57,20 @@ int launch_reduce_core(const T* x, int64_t n, typename Op::acc_t init, typename 
                        partials);
     RC_HIP(hipGetLastError());
   }
+  const typename Op::acc_t* vals = partials;
+  int64_t count = ntiles;
+  if (split_first_level<Op>(ntiles))
+  {
+    const int64_t groups = rc::ceil_div(ntiles, (int64_t)64);
+    hipLaunchKernelGGL((rt::group_kernel<Op>), dim3((unsigned)groups), dim3(64), 0, s, partials,
+                       ntiles, partials + ntiles);
+    RC_HIP(hipGetLastError());
+    vals = partials + ntiles;
+    count = groups;
+  }
   hipLaunchKernelGGL((rt::final_kernel<Op, kFinalWaves, kFinalMaxGroups>), dim3(1),
-                     dim3(kFinalWaves * 64), 0, s, partials, (const uint32_t*)nullptr,
-                     (const uint32_t*)nullptr, ntiles, init, out, (const int64_t*)nullptr);
+                     dim3(kFinalWaves * 64), 0, s, vals, (const uint32_t*)nullptr,
+                     (const uint32_t*)nullptr, count, init, out, (const int64_t*)nullptr);
   RC_HIP(hipGetLastError());
   return RAJA_CDNA4_SUCCESS;
 }
@@ -60,7 +90,7 @@ int launch_reduce(const T* x, int64_t n, typename Op::acc_t init,
   {
     return rc::set_error(RAJA_CDNA4_ERROR_UNSUPPORTED, "span too long for one reduction");
   }
-  RC_TRY(rc::ensure_tmp(c, (size_t)(ntiles > 0 ? ntiles : 1) * sizeof(acc_t)));
+  RC_TRY(rc::ensure_tmp(c, (size_t)(ntiles > 0 ? partial_slots<Op>(ntiles) : 1) * sizeof(acc_t)));
   return launch_reduce_core<T, Op>(x, n, init, out, (acc_t*)c->tmp, s, name);
 }
 

@@ -212,6 +212,23 @@ __global__ __launch_bounds__(kBlock) void tiles_kernel(const T* __restrict__ x, 
   }
 }
 
+// First level of the 64-ary tree on its own (one 64-lane block per group of
+// 64 tile values): group g's value is the wave fold of values 64g..64g+63,
+// exactly as tree_fold's first level computes it, so folding the group
+// values with tree_fold gives the bits of folding the tile values.  Used
+// for wide accumulators, whose wave fold is too slow to run group after
+// group in one block.
+template<typename Op>
+__global__ __launch_bounds__(64) void group_kernel(const typename Op::acc_t* __restrict__ in,
+                                                   int64_t count,
+                                                   typename Op::acc_t* __restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * wave_size + threadIdx.x;
+  const typename Op::acc_t v = (i < count) ? in[i] : Op::ident();
+  const typename Op::acc_t r = wave_reduce(v, Op{});
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
 // Kernel 2: the 64-ary tree over the tile values of each segment (one
 // block per segment; a span reduction is one segment) and the init;
 // segment s's result goes to out[out_index[s]] (out[s] without an index).
