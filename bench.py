@@ -170,7 +170,7 @@ def main():
                     % (k, fa[k]["warm_ms"], fa[k]["warm_GBs"], fa[k]["cold_ms"], fa[k]["cold_GBs"]))
         fm = run_example("forall_multi_reducers", [str(args.log2n), "10"])
         if fm:
-            for k in ("reducers", "params"):
+            for k in ("reducers", "params", "reducers_nontemporal_body"):
                 log("bench: forall Sum/Min/Max/MinLoc %-8s %.4f ms (%.1f GB/s, %.1f%% of 8 TB/s)"
                     % (k, fm[k]["ms"], fm[k]["GBs"], 100 * fm[k]["hbm_frac"]))
 

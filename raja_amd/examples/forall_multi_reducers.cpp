@@ -6,13 +6,15 @@
 // include/RAJA/policy/hip/reduce.hpp:960-1161).  One read of x, 8 B per
 // element.
 //
-// The loop runs twice per repetition: with old-style reducers (get()
-// synchronises) and with four expt::Reduce parameters (resolved at the end
-// of forall).  x comes from the library's counter generator (normal, seed 5).
-// The host checks the last repetition: min, max and the first index of the
-// minimum exactly, the sum within 2*ceil(log2 N)*eps*sum|x| of a compensated
-// (Kahan-Neumaier) sum, and the two reducer styles bit for bit.  Prints one
-// JSON line.
+// The loop runs three times per repetition: with old-style reducers (get()
+// synchronises), with four expt::Reduce parameters (resolved at the end of
+// forall), and with old-style reducers and a body that reads x through a
+// nontemporal load (a read-once hint outside RAJA's API, reported apart).
+// x comes from the library's counter generator (normal, seed 5).  The host
+// checks the last repetition: min, max and the first index of the minimum
+// exactly, the sum within 2*ceil(log2 N)*eps*sum|x| of a compensated
+// (Kahan-Neumaier) sum, and the three runs bit for bit.  Prints one JSON
+// line.
 //
 // Usage: forall_multi_reducers [log2n] [reps]     (default 28 10)
 #include <algorithm>
@@ -85,8 +87,8 @@ int main(int argc, char** argv)
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
-  std::vector<double> t_old, t_par, h_old, h_par;
-  Result r_old, r_par;
+  std::vector<double> t_old, t_par, t_nt, h_old, h_par;
+  Result r_old, r_par, r_nt;
 
   for (int r = 0; r < reps + 1; ++r)
   {
@@ -157,6 +159,33 @@ int main(int argc, char** argv)
         h_par.push_back(1e3 * hs);
       }
     }
+    check(hipStreamSynchronize(st), "sync");
+    {
+      // The old-style loop again, its body reading x with a nontemporal load
+      // (read-once data: it bypasses the caches' allocation, as the library's
+      // own reduction kernels do).  Not RAJA API: it shows what the same loop
+      // reaches when the body marks its stream.  Same values, same bits.
+      RAJA::ReduceSum<RED, double> sum(0.0);
+      RAJA::ReduceMin<RED, double> mn(std::numeric_limits<double>::max());
+      RAJA::ReduceMax<RED, double> mx(-std::numeric_limits<double>::max());
+      RAJA::ReduceMinLoc<RED, double> mloc;
+      check(hipEventRecord(e0, st), "record");
+      RAJA::forall<EXEC>(res, RAJA::RangeSegment(0, N), [=] RAJA_DEVICE(RAJA::Index_type i) {
+        const double v = __builtin_nontemporal_load(&x[i]);
+        sum += v;
+        mn.min(v);
+        mx.max(v);
+        mloc.minloc(v, i);
+      });
+      check(hipEventRecord(e1, st), "record");
+      r_nt.sum = sum.get();
+      r_nt.mn = mn.get();
+      r_nt.mx = mx.get();
+      r_nt.lv = mloc.get();
+      r_nt.loc = mloc.getLoc();
+      check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      if (keep) t_nt.push_back(ms);
+    }
   }
 
   std::vector<double> hx(N);
@@ -181,18 +210,22 @@ int main(int argc, char** argv)
   };
   const bool same = r_old.sum == r_par.sum && r_old.mn == r_par.mn && r_old.mx == r_par.mx &&
                     r_old.lv == r_par.lv && r_old.loc == r_par.loc;
-  const bool ok = good(r_old) && good(r_par) && same;
+  const bool same_nt = r_old.sum == r_nt.sum && r_old.mn == r_nt.mn && r_old.mx == r_nt.mx &&
+                       r_old.lv == r_nt.lv && r_old.loc == r_nt.loc;
+  const bool ok = good(r_old) && good(r_par) && same && same_nt;
   const double bytes = 8.0 * N;
   auto gbs = [&](double ms) { return bytes / (ms * 1e-3) / 1e9; };
   std::printf("{\"n\": %lld, \"reps\": %d, \"pass\": %s, \"styles_bit_identical\": %s, \"sum\": %.17g, "
               "\"sum_abs_err_vs_kn\": %.3g, \"sum_bound\": %.3g, \"min\": %.17g, \"max\": %.17g, "
               "\"minloc\": [%.17g, %lld], "
               "\"reducers\": {\"ms\": %.4f, \"GBs\": %.1f, \"hbm_frac\": %.4f, \"host_ms_incl_get\": %.4f}, "
-              "\"params\": {\"ms\": %.4f, \"GBs\": %.1f, \"hbm_frac\": %.4f, \"host_ms_incl_resolve\": %.4f}}\n",
+              "\"params\": {\"ms\": %.4f, \"GBs\": %.1f, \"hbm_frac\": %.4f, \"host_ms_incl_resolve\": %.4f}, "
+              "\"reducers_nontemporal_body\": {\"ms\": %.4f, \"GBs\": %.1f, \"hbm_frac\": %.4f, \"same_bits\": %s}}\n",
               (long long)N, reps, ok ? "true" : "false", same ? "true" : "false", r_old.sum,
               std::fabs(r_old.sum - kn), bound, r_old.mn, r_old.mx, r_old.lv, (long long)r_old.loc,
               median(t_old), gbs(median(t_old)), gbs(median(t_old)) / 8000.0, median(h_old), median(t_par),
-              gbs(median(t_par)), gbs(median(t_par)) / 8000.0, median(h_par));
+              gbs(median(t_par)), gbs(median(t_par)) / 8000.0, median(h_par), median(t_nt),
+              gbs(median(t_nt)), gbs(median(t_nt)) / 8000.0, same_nt ? "true" : "false");
   check(hipFree(x), "free");
   return ok ? 0 : 3;
 }

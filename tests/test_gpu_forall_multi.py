@@ -1,8 +1,8 @@
 """One forall with ReduceSum/Min/Max/MinLoc over FP64 N(0,1)
 (raja_amd/examples/forall_multi_reducers.cpp) on the GPU: min, max and the
 first index of the minimum exact, the sum within 2*ceil(log2 N)*eps*sum|x| of
-a compensated host sum, old-style reducers and expt::Reduce parameters bit
-for bit identical."""
+a compensated host sum, old-style reducers, expt::Reduce parameters and the
+loop whose body reads x through a nontemporal load bit for bit identical."""
 import json
 import os
 import subprocess
@@ -24,5 +24,6 @@ def test_forall_multi_reducers(log2n):
     assert r.returncode == 0, r.stdout + r.stderr
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["pass"] and out["styles_bit_identical"]
+    assert out["reducers_nontemporal_body"]["same_bits"]
     assert out["sum_abs_err_vs_kn"] <= out["sum_bound"]
     assert out["minloc"][0] == out["min"]
