@@ -87,11 +87,13 @@ struct Arena
   }
 };
 
-// A unit slot: the pinned staging of a batch's tables and the completion
-// event.  A slot is reused only after its unit retired.
+// A unit slot: the pinned staging of a batch's tables, the event recorded
+// after their upload and the completion event.  A slot is reused only after
+// its unit retired.
 struct Slot
 {
   hipEvent_t done = nullptr;
+  hipEvent_t uploaded = nullptr;
   char* staging = nullptr;
   size_t staging_cap = 0;
 };
@@ -346,6 +348,10 @@ struct raja_cdna4_server_s
   int device = 0;
   Arena arena;
   std::vector<hipStream_t> streams;
+  // Batched mode: the stream that uploads batch tables, so a unit's upload
+  // runs while the units before it compute instead of between them on
+  // their stream.
+  hipStream_t upload = nullptr;
   std::vector<Slot> slots;
   BatchPlan plan;
 
@@ -395,7 +401,9 @@ int launch_batch(raja_cdna4_server_s* sv, Slot& slot, char* dev, hipStream_t s)
       slot.staging_cap = cap;
     }
     p.write(slot.staging);
-    RC_HIP(hipMemcpyAsync(dev, slot.staging, p.rec_bytes, hipMemcpyHostToDevice, s));
+    RC_HIP(hipMemcpyAsync(dev, slot.staging, p.rec_bytes, hipMemcpyHostToDevice, sv->upload));
+    RC_HIP(hipEventRecord(slot.uploaded, sv->upload));
+    RC_HIP(hipStreamWaitEvent(s, slot.uploaded, 0));
   }
   if (!p.sum_out.empty())
   {
@@ -478,6 +486,7 @@ struct Scheduler
       {
         const int st = rc::hip_error(q, "hipEventQuery", __FILE__, __LINE__);
         for (hipStream_t s : sv->streams) (void)hipStreamSynchronize(s);
+        if (sv->upload) (void)hipStreamSynchronize(sv->upload);
         for (Unit& u : inflight)
         {
           fail_unit(u, st, finish);
@@ -581,6 +590,7 @@ struct Scheduler
       if (st != RAJA_CDNA4_SUCCESS)
       {
         (void)hipStreamSynchronize(s);
+        if (sv->upload) (void)hipStreamSynchronize(sv->upload);
         fail_unit(u, st, finish);
         sv->arena.release(off, bytes);
         free_slots.push_back(slot);
@@ -702,9 +712,11 @@ int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_s
   sv->cfg = c;
   auto fail = [&](hipError_t e, const char* what) {
     for (hipStream_t s : sv->streams) (void)hipStreamDestroy(s);
+    if (sv->upload) (void)hipStreamDestroy(sv->upload);
     for (Slot& sl : sv->slots)
     {
       if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.uploaded) (void)hipEventDestroy(sl.uploaded);
     }
     if (sv->arena.base) (void)hipFree(sv->arena.base);
     delete sv;
@@ -730,10 +742,17 @@ int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_s
     if (e != hipSuccess) return fail(e, "hipStreamCreateWithFlags");
     sv->streams.push_back(s);
   }
+  if (c.mode == RAJA_CDNA4_SERVE_BATCHED)
+  {
+    e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(e, "hipStreamCreateWithFlags");
+  }
   sv->slots.resize((size_t)c.max_inflight);
   for (Slot& sl : sv->slots)
   {
     e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(e, "hipEventCreateWithFlags");
+    e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
     if (e != hipSuccess) return fail(e, "hipEventCreateWithFlags");
   }
   *out = sv;
@@ -848,10 +867,13 @@ int raja_cdna4_server_destroy(raja_cdna4_server_s* sv)
   if (!sv) return RAJA_CDNA4_SUCCESS;
   RC_TRY(raja_cdna4_server_stop(sv));
   for (hipStream_t s : sv->streams) RC_HIP(hipStreamSynchronize(s));
+  if (sv->upload) RC_HIP(hipStreamSynchronize(sv->upload));
   for (hipStream_t s : sv->streams) RC_HIP(hipStreamDestroy(s));
+  if (sv->upload) RC_HIP(hipStreamDestroy(sv->upload));
   for (Slot& sl : sv->slots)
   {
     if (sl.done) RC_HIP(hipEventDestroy(sl.done));
+    if (sl.uploaded) RC_HIP(hipEventDestroy(sl.uploaded));
     if (sl.staging) RC_HIP(hipHostFree(sl.staging));
   }
   if (sv->arena.base) RC_HIP(hipFree(sv->arena.base));

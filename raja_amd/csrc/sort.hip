@@ -59,6 +59,46 @@ __global__ void clear_words_kernel(uint32_t* p, int64_t n)
 }
 
 // ---- 1. all-pass digit histograms ------------------------------------------
+// Counts the N keys every lane of the wave holds (all of them valid) into
+// s_hist, bin b's copy c at word b * C + c, lane l counting into copy l % C.
+// A digit that is the same for all 64 * N keys is added once: the test is
+// an OR over the keys of (bits ^ lane 0's first bits), one ballot per digit.
+template<typename K, bool DESC, int P, int N, int C>
+__device__ __forceinline__ void count_full_wave(const K (&k)[N], uint32_t* s_hist, int lane)
+{
+  using bits_t = typename KeyTraits<K>::bits_t;
+  const bits_t b0 = KeyTraits<K>::to_bits(k[0]);
+  bits_t f = 0;
+  if constexpr (sizeof(bits_t) == 4)
+  {
+    f = (bits_t)__builtin_amdgcn_readfirstlane((int)b0);
+  }
+  else
+  {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
+    f = ((bits_t)hi << 32) | lo;
+  }
+  bits_t x = 0;
+#pragma unroll
+  for (int q = 0; q < N; ++q) x |= KeyTraits<K>::to_bits(k[q]) ^ f;
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+  {
+    if (ballot(((x >> (8 * p)) & (bits_t)0xFF) != 0) == 0ull)
+    {
+      const uint32_t d0 = (uint32_t)((DESC ? ~f : f) >> (8 * p)) & (kRadix - 1);
+      if (lane == 0) atomicAdd(&s_hist[(p * kRadix + d0) * C], (uint32_t)(wave_size * N));
+    }
+    else
+    {
+      uint32_t* const h = s_hist + p * kRadix * C + (lane % C);
+#pragma unroll
+      for (int q = 0; q < N; ++q) atomicAdd(&h[digit_of<K, DESC>(k[q], 8 * p) * C], 1u);
+    }
+  }
+}
+
 // Each wave streams 64 lanes x U x 16-byte vectors per iteration (U vector
 // loads in flight per lane) and counts every digit of every key with LDS
 // atomics; a digit shared by all keys of a full iteration is added once
@@ -117,40 +157,7 @@ __global__ __launch_bounds__(256) void hist_kernel(const K* __restrict__ keys, i
     }
     if (full)
     {
-      // A digit that is the same for all U*VEC*64 keys of the iteration is
-      // added once; the others are counted key by key.  The test: OR over
-      // the keys of (bits ^ lane 0's first bits), one ballot per digit.
-      using bits_t = typename KeyTraits<K>::bits_t;
-      const bits_t b0 = KeyTraits<K>::to_bits(k[0]);
-      bits_t f = 0;
-      if constexpr (sizeof(bits_t) == 4)
-      {
-        f = (bits_t)__builtin_amdgcn_readfirstlane((int)b0);
-      }
-      else
-      {
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
-        f = ((bits_t)hi << 32) | lo;
-      }
-      bits_t x = 0;
-#pragma unroll
-      for (int q = 0; q < U * VEC; ++q) x |= KeyTraits<K>::to_bits(k[q]) ^ f;
-#pragma unroll
-      for (int p = 0; p < P; ++p)
-      {
-        if (ballot(((x >> (8 * p)) & (bits_t)0xFF) != 0) == 0ull)
-        {
-          const uint32_t d0 = (uint32_t)((DESC ? ~f : f) >> (8 * p)) & (kRadix - 1);
-          if (lane == 0) atomicAdd(&s_hist[(p * kRadix + d0) * kCopies], (uint32_t)PER_WAVE);
-        }
-        else
-        {
-          uint32_t* const h = s_hist + p * kRadix * kCopies + (lane % kCopies);
-#pragma unroll
-          for (int q = 0; q < U * VEC; ++q) atomicAdd(&h[digit_of<K, DESC>(k[q], 8 * p) * kCopies], 1u);
-        }
-      }
+      count_full_wave<K, DESC, P, U * VEC, kCopies>(k, s_hist, lane);
       continue;
     }
 #pragma unroll
@@ -626,63 +633,120 @@ struct SortWs
 };
 
 // ---- segmented onesweep: the tiles of many ranges, one launch per pass ----------
-// Digit histograms of every range of a tile table: one block per tile adds
-// its keys' digit counts to its range's P rows of 256 counters, and clears
-// the tile's descriptor row of the first pass.  Keys are read 8 per thread
-// at a time (a tile starts at any element: 4-byte loads), and counted into
-// 8 side-by-side copies of each bin as in hist_kernel (the bench's batched
-// serving: 14.2 -> 9.0 ms of histogram time over the same batches, against
-// one 4-byte load per thread per step into a single histogram).
+// Consecutive tile records one histogram block takes at most (a range's
+// records are consecutive, so a block adds its counts to the global rows
+// once per range it meets instead of once per tile), and the blocks a launch
+// keeps at least before it gives a block more than one record: a block of
+// 4 records takes about 30 us, so small tables take one record per block.
+constexpr int kSegHistTiles = 4;
+constexpr int64_t kSegHistMinBlocks = 2048;
+
+// Digit histograms of every range of a tile table: block b takes records
+// per_block * b .. + per_block - 1, counts their keys' digits in LDS
+// and adds the counts to the range's P rows of 256 counters when the range
+// changes or its records end; it clears each record's descriptor row of the
+// first pass.  Keys are read as 16-byte vectors at the element's alignment
+// (a tile starts at any element), 4 in flight per thread, and counted as in
+// hist_kernel (8 side-by-side copies of each bin, a digit uniform over a
+// wave's full iteration added once).  The bench's batched serving spent
+// 14.2 ms in this kernel with one 4-byte load per thread per step and one
+// histogram, 9.0 ms with 8 keys in flight and the side-by-side copies, and
+// each of those blocks took one record.
 template<typename K, bool DESC, int P>
 __global__ __launch_bounds__(256) void seg_hist_kernel(const K* __restrict__ keys,
                                                        const rs::SortTile* __restrict__ recs,
+                                                       int64_t ntiles, int per_block,
                                                        int64_t tile_elems,
                                                        uint32_t* __restrict__ hist,
                                                        uint32_t* __restrict__ desc0)
 {
+  static_assert(sizeof(K) == 4, "the segmented onesweep takes 4-byte keys");
   constexpr int kCopies = 8;
-  constexpr int UNR = 8;
-  __shared__ uint32_t s_hist[P * kRadix * kCopies];
-  for (int i = threadIdx.x; i < P * kRadix * kCopies; i += 256) s_hist[i] = 0u;
-  const rs::SortTile r = recs[blockIdx.x];
+  constexpr int VEC = 4;
+  constexpr int U = 4;
+  constexpr int kBins = P * kRadix * kCopies;
+  typedef K uvec_t __attribute__((ext_vector_type(VEC), aligned(sizeof(K))));
+  __shared__ uint32_t s_hist[kBins];
+  for (int i = threadIdx.x; i < kBins; i += 256) s_hist[i] = 0u;
   __syncthreads();
-  const int64_t first = (int64_t)r.j * tile_elems;
-  const int64_t rem = r.len - first;
-  const int cnt = (int)(rem < tile_elems ? rem : tile_elems);
-  const K* kp = keys + r.ik + first;
-  uint32_t* const hc = s_hist + (lane_id() % kCopies);
-  int i = threadIdx.x;
-  for (; i + (UNR - 1) * 256 < cnt; i += UNR * 256)
+  const int lane = lane_id();
+  const int64_t t0 = (int64_t)blockIdx.x * per_block;
+  const int64_t t1 = (t0 + per_block < ntiles) ? t0 + per_block : ntiles;
+  for (int64_t t = t0; t < t1; ++t)
   {
-    K k[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) k[u] = kp[i + u * 256];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u)
+    const rs::SortTile r = recs[t];
+    const int64_t first = (int64_t)r.j * tile_elems;
+    const int64_t rem = r.len - first;
+    const int cnt = (int)(rem < tile_elems ? rem : tile_elems);
+    const K* const kp = keys + r.ik + first;
+    const int nvec = cnt / VEC;
+    // waves step through the tile's vectors in runs of U * 256
+    for (int base = 0; base < nvec; base += U * 256)
     {
-#pragma unroll
-      for (int p = 0; p < P; ++p)
+      K k[U * VEC];
+      if (base + U * 256 <= nvec)
       {
-        atomicAdd(&hc[(p * kRadix + digit_of<K, DESC>(k[u], 8 * p)) * kCopies], 1u);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+          const uvec_t x = __builtin_nontemporal_load(
+              reinterpret_cast<const uvec_t*>(kp + (int64_t)(base + u * 256 + (int)threadIdx.x) * VEC));
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) k[u * VEC + e] = x[e];
+        }
+        count_full_wave<K, DESC, P, U * VEC, kCopies>(k, s_hist, lane);
+      }
+      else
+      {
+        uint32_t* const hc = s_hist + lane % kCopies;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+          const int v = base + u * 256 + (int)threadIdx.x;
+          if (v < nvec)
+          {
+            const uvec_t x = *reinterpret_cast<const uvec_t*>(kp + (int64_t)v * VEC);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+            {
+#pragma unroll
+              for (int p = 0; p < P; ++p)
+              {
+                atomicAdd(&hc[(p * kRadix + digit_of<K, DESC>(x[e], 8 * p)) * kCopies], 1u);
+              }
+            }
+          }
+        }
       }
     }
-  }
-  for (; i < cnt; i += 256)
-  {
-    const K k = kp[i];
+    // the last cnt % VEC keys
+    if ((int)threadIdx.x < cnt - nvec * VEC)
+    {
+      const K key = kp[(int64_t)nvec * VEC + threadIdx.x];
+      uint32_t* const hc = s_hist + lane % kCopies;
 #pragma unroll
-    for (int p = 0; p < P; ++p) atomicAdd(&hc[(p * kRadix + digit_of<K, DESC>(k, 8 * p)) * kCopies], 1u);
-  }
-  __syncthreads();
-  uint32_t* const h = hist + (int64_t)r.seg * P * kRadix;
-  for (int b = threadIdx.x; b < P * kRadix; b += 256)
-  {
-    uint32_t c = 0;
+      for (int p = 0; p < P; ++p) atomicAdd(&hc[(p * kRadix + digit_of<K, DESC>(key, 8 * p)) * kCopies], 1u);
+    }
+    for (int i = threadIdx.x; i < kRowStride; i += 256) desc0[t * kRowStride + i] = 0u;
+    if (t + 1 == t1 || recs[t + 1].seg != r.seg)
+    {
+      // add the range's counts to its rows and clear them for the next range
+      __syncthreads();
+      uint32_t* const h = hist + (int64_t)r.seg * P * kRadix;
+      for (int b = threadIdx.x; b < P * kRadix; b += 256)
+      {
+        uint32_t c = 0;
 #pragma unroll
-    for (int q = 0; q < kCopies; ++q) c += s_hist[b * kCopies + q];
-    if (c) atomic_add_agent(&h[b], c);
+        for (int q = 0; q < kCopies; ++q)
+        {
+          c += s_hist[b * kCopies + q];
+          s_hist[b * kCopies + q] = 0u;
+        }
+        if (c) atomic_add_agent(&h[b], c);
+      }
+      __syncthreads();
+    }
   }
-  for (int i = threadIdx.x; i < kRowStride; i += 256) desc0[(int64_t)blockIdx.x * kRowStride + i] = 0u;
 }
 
 // Sorts every range of a tile table (records built with tile SortShape::TILE)
@@ -706,8 +770,10 @@ int seg_sort_core(const K* in_k, const V* in_v, K* out_k, V* out_v, K* alt_k, V*
   const int64_t cb = rc::ceil_div(words, 256);
   hipLaunchKernelGGL(clear_words_kernel, dim3((unsigned)(cb < 1024 ? cb : 1024)), dim3(256), 0, s,
                      hist, words);
-  hipLaunchKernelGGL((seg_hist_kernel<K, DESC, P>), dim3((unsigned)ntiles), dim3(256), 0, s, in_k,
-                     d_recs, S::TILE, hist, desc);
+  const int per_block =
+      (int)std::max<int64_t>(1, std::min<int64_t>(kSegHistTiles, ntiles / kSegHistMinBlocks));
+  hipLaunchKernelGGL((seg_hist_kernel<K, DESC, P>), dim3((unsigned)rc::ceil_div(ntiles, (int64_t)per_block)),
+                     dim3(256), 0, s, in_k, d_recs, ntiles, per_block, S::TILE, hist, desc);
   RC_HIP(hipGetLastError());
   uint32_t* const d0 = desc;
   uint32_t* const d1 = desc + ntiles * kRowStride;

@@ -198,13 +198,18 @@ def test_segmented_scan_f64_matches_span_scans(raja):
     plan.close()
 
 
-@pytest.mark.parametrize("kdt,with_vals,desc", [(torch.int32, True, False), (torch.float32, False, True),
-                                                (torch.uint32, False, False), (torch.float32, True, False)])
-def test_segmented_sort_long_segments_one_pass_per_digit(raja, kdt, with_vals, desc):
+@pytest.mark.parametrize("kdt,with_vals,desc,small", [(torch.int32, True, False, False),
+                                                      (torch.float32, False, True, False),
+                                                      (torch.uint32, False, False, False),
+                                                      (torch.float32, True, False, False),
+                                                      (torch.uint32, True, True, True)])
+def test_segmented_sort_long_segments_one_pass_per_digit(raja, kdt, with_vals, desc, small):
     """Segments longer than one block sort (8193 to 150000 keys, mixed with
     short and empty ones) are sorted together by the segmented onesweep;
     each must equal numpy's stable sort of its own keys (ascending, or the
-    stable descending order)."""
+    stable descending order).  Small keys (below 2^12) make the high digits
+    the same over whole waves of the histogram kernel (counted once per
+    wave)."""
     rng = np.random.default_rng(12)
     lens = np.concatenate([rng.integers(8193, 150000, 40), [0, 1, 100, 8192, 8193, 16384, 16385],
                            rng.integers(0, 8192, 60)])
@@ -219,13 +224,13 @@ def test_segmented_sort_long_segments_one_pass_per_digit(raja, kdt, with_vals, d
     elif kdt == torch.int32:
         hk = rng.integers(-3000, 3000, n).astype(np.int32)
     else:
-        hk = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        hk = rng.integers(0, 1 << (12 if small else 32), n, dtype=np.uint64).astype(np.uint32)
     k = torch.from_numpy(hk.view(np.int32)).to(DEV).view(kdt) if kdt == torch.uint32 \
         else torch.from_numpy(hk).to(DEV)
     v = torch.arange(n, dtype=torch.int32, device=DEV) if with_vals else None
     plan = raja.Segments(off)
-    print("segmented onesweep: %d keys in %d segments, %s, vals=%s, desc=%s"
-          % (n, len(lens), kdt, with_vals, desc), flush=True)
+    print("segmented onesweep: %d keys in %d segments, %s, vals=%s, desc=%s, small=%s"
+          % (n, len(lens), kdt, with_vals, desc, small), flush=True)
     raja.segmented_sort_pairs(plan, k, v, raja.operators.greater if desc else raja.operators.less)
     torch.cuda.synchronize()
     gk = (k.view(torch.int32).cpu().numpy().view(np.uint32) if kdt == torch.uint32
