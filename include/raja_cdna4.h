@@ -300,7 +300,8 @@ int raja_cdna4_segmented_sort_pairs(raja_cdna4_segments plan, int key_dtype, voi
  * Every request's result is bit-identical whether it runs alone, in a batch
  * or in either mode (sums use the canonical reduction tree; scans and sorts
  * are exact).  Request inputs must not alias any output of a request that
- * may be in flight at the same time. */
+ * may be in flight at the same time, and such requests' outputs must not
+ * overlap: requests in flight run in no fixed order. */
 
 enum raja_cdna4_request_kind
 {

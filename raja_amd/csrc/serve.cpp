@@ -87,13 +87,14 @@ struct Arena
   }
 };
 
-// A unit slot: the pinned staging of a batch's tables, the event recorded
-// after their upload and the completion event.  A slot is reused only after
-// its unit retired.
+// A unit slot: the pinned staging of a batch's tables, the events recorded
+// after their upload and after the batch's block sorts, and the completion
+// event.  A slot is reused only after its unit retired.
 struct Slot
 {
   hipEvent_t done = nullptr;
   hipEvent_t uploaded = nullptr;
+  hipEvent_t sorted = nullptr;
   char* staging = nullptr;
   size_t staging_cap = 0;
 };
@@ -405,6 +406,18 @@ int launch_batch(raja_cdna4_server_s* sv, Slot& slot, char* dev, hipStream_t s)
     RC_HIP(hipEventRecord(slot.uploaded, sv->upload));
     RC_HIP(hipStreamWaitEvent(s, slot.uploaded, 0));
   }
+  // The block sorts (a few dozen blocks, about 35 us) run on the upload
+  // stream, after the upload, beside the unit's other kernels; the unit's
+  // stream waits for them at its end.
+  const bool side = !p.sorts.empty() && (!p.sum_out.empty() || !p.scan_tiles.empty() ||
+                                         !p.sort_tiles.empty());
+  if (!p.sorts.empty())
+  {
+    hipStream_t bs = side ? sv->upload : s;
+    RC_TRY(rws::block_sort_u32_i32((const uint32_t*)p.sort_base, (const rs::SortSeg*)(dev + p.o_sorts),
+                                   (int64_t)p.sorts.size(), bs));
+    if (side) RC_HIP(hipEventRecord(slot.sorted, bs));
+  }
   if (!p.sum_out.empty())
   {
     RC_TRY(rws::table_sum_f64((const double*)p.sum_base, (const rs::TileRec*)(dev + p.o_sum_tiles),
@@ -418,11 +431,6 @@ int launch_batch(raja_cdna4_server_s* sv, Slot& slot, char* dev, hipStream_t s)
                                     (const rs::TileRec*)(dev + p.o_scan_tiles),
                                     (int64_t)p.scan_tiles.size(), dev + p.o_desc, s));
   }
-  if (!p.sorts.empty())
-  {
-    RC_TRY(rws::block_sort_u32_i32((const uint32_t*)p.sort_base, (const rs::SortSeg*)(dev + p.o_sorts),
-                                   (int64_t)p.sorts.size(), s));
-  }
   if (!p.sort_tiles.empty())
   {
     void* base = (void*)p.sort_base;
@@ -431,6 +439,7 @@ int launch_batch(raja_cdna4_server_s* sv, Slot& slot, char* dev, hipStream_t s)
                                  (int64_t)p.sort_tiles.size(), p.sort_ranges,
                                  (uint32_t*)(dev + p.o_hist), (uint32_t*)(dev + p.o_sdesc), s));
   }
+  if (side) RC_HIP(hipStreamWaitEvent(s, slot.sorted, 0));
   return RAJA_CDNA4_SUCCESS;
 }
 
@@ -717,6 +726,7 @@ int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_s
     {
       if (sl.done) (void)hipEventDestroy(sl.done);
       if (sl.uploaded) (void)hipEventDestroy(sl.uploaded);
+      if (sl.sorted) (void)hipEventDestroy(sl.sorted);
     }
     if (sv->arena.base) (void)hipFree(sv->arena.base);
     delete sv;
@@ -753,6 +763,8 @@ int raja_cdna4_server_create(const raja_cdna4_serve_config* config, raja_cdna4_s
     e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e != hipSuccess) return fail(e, "hipEventCreateWithFlags");
     e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(e, "hipEventCreateWithFlags");
+    e = hipEventCreateWithFlags(&sl.sorted, hipEventDisableTiming);
     if (e != hipSuccess) return fail(e, "hipEventCreateWithFlags");
   }
   *out = sv;
@@ -874,6 +886,7 @@ int raja_cdna4_server_destroy(raja_cdna4_server_s* sv)
   {
     if (sl.done) RC_HIP(hipEventDestroy(sl.done));
     if (sl.uploaded) RC_HIP(hipEventDestroy(sl.uploaded));
+    if (sl.sorted) RC_HIP(hipEventDestroy(sl.sorted));
     if (sl.staging) RC_HIP(hipHostFree(sl.staging));
   }
   if (sv->arena.base) RC_HIP(hipFree(sv->arena.base));
