@@ -517,6 +517,24 @@ def serving(R, torch, np, dev, nreq=4000):
                                   "p50_ms": round(p50, 4), "p99_ms": round(p99, 4)}
         log("bench: serve poisson %.0f req/s %-12s p50 %.3f ms  p99 %.3f ms" % (lam, name, p50, p99))
         srv.close()
+
+    # The batched server's latency over a sweep of offered rates (fractions
+    # of its saturation rate): where p99 turns up is the knee.
+    srv = R.Server(R.SERVE_BATCHED, 4, 8, 512, 1 << 24, 64 << 30, True)
+    sweep = []
+    for frac in (0.2, 0.4, 0.6, 0.8, 0.9, 1.0):
+        lam_f = frac * sat["batched"]
+        arr_f = np.cumsum(np.random.default_rng(11).exponential(1.0 / lam_f, nreq))
+        reqs_f, _ = build(kinds, sizes, np.random.default_rng(9), arr_f)
+        res, wall = srv.run(reqs_f)
+        assert all(r.status == 0 for r in res)
+        p50, p99 = lat(res, arr_f)
+        sweep.append({"fraction_of_saturation": frac, "offered_req_per_s": round(lam_f, 1),
+                      "req_per_s": round(nreq / wall, 1), "p50_ms": round(p50, 4), "p99_ms": round(p99, 4)})
+        log("bench: serve poisson sweep %.1f x saturation (%.0f req/s) batched p50 %.3f ms  p99 %.3f ms"
+            % (frac, lam_f, p50, p99))
+    srv.close()
+    out["poisson_batched_sweep"] = sweep
     return out
 
 
