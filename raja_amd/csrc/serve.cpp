@@ -1,5 +1,5 @@
 // Request server: arena-backed admission, coalescing of small requests into
-// batched launches, large requests alone on round-robin streams.
+// batched launches, large requests alone.
 //
 // The reference serves a request with one algorithm call, whose temporaries
 // come from a pool (include/RAJA/util/basic_mempool.hpp:54-454) and whose
@@ -16,6 +16,10 @@
 //     (tile records carry each request's output offset), sorts of up to
 //     8192 pairs with one block per request in LDS and longer ones with the
 //     segmented onesweep (one launch per digit pass over all their tiles);
+//   * units with scans or radix passes run on stream 0 (lookback launches
+//     run one at a time on the device anyway), the others round-robin over
+//     the remaining streams; a batch's tables upload on a stream of their
+//     own, so an upload overlaps the units before it;
 //   * completion is observed by polling one event per unit, so new arrivals
 //     are admitted while earlier units run.
 #include <string.h>
