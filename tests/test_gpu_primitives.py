@@ -336,6 +336,29 @@ def test_sort_pairs_types_descending(raja, orc, kdt, vdt):
         check_stable_pairs(hk, to_np(kk), to_np(v), desc)
 
 
+@pytest.mark.parametrize("bits", [0, 8, 24])
+def test_sort_pairs_skips_constant_digits(raja, orc, bits):
+    """From 2^18 keys the digit passes whose digit is the same for every key
+    are skipped: equal keys run none, 8-bit keys one and 24-bit keys three
+    (an odd count, which in place ends with a copy back into the keys);
+    every result is the stable sort."""
+    for n in ((1 << 18) + 5, (1 << 20) + 3):
+        print("sort skipping constant digits: n %d, %d-bit keys" % (n, bits), flush=True)
+        k = torch.empty(n, dtype=torch.uint32, device=DEV)
+        if bits == 0:
+            raja.fill(raja.FILL_U32_CONST, k, a=77)
+        else:
+            raja.fill(raja.FILL_U32_BITS, k, seed=5, a=bits)
+        v = torch.arange(n, dtype=torch.int32, device=DEV)
+        hk = to_np(k)
+        raja.sort_pairs(k, v)
+        ek, ev = orc.seq_stable_sort_pairs(hk, np.arange(n, dtype=np.int32))
+        assert np.array_equal(to_np(k), ek), (n, bits)
+        assert np.array_equal(to_np(v), ev), (n, bits)
+        raja.sort(k, raja.operators.greater)
+        assert np.array_equal(to_np(k), np.sort(hk)[::-1]), (n, bits)
+
+
 def test_sort_keys_only(raja, orc):
     for n in (17, 4097, 16385, 300001):
         k = torch.empty(n, dtype=torch.uint32, device=DEV)

@@ -360,3 +360,38 @@ def test_open_server_producer_threads(raja, orc):
     srv.close()
     assert sorted(got) == sorted(tks) and all(v == 0 for v in got.values())
     check_against_oracle(raja, orc, pools, pools.snapshot())
+
+
+def test_serve_large_sorts_skip_constant_digits(raja, orc):
+    """Sorts of 2^18 + 5 pairs served alone (the out-of-place span sort)
+    with keys whose high digits are constant, down to all keys equal: every
+    result lands in the request's own output buffers as the stable sort, and
+    the inputs are left as they were."""
+    n = (1 << 18) + 5
+    bits_cases = [0, 8, 16, 24, 32]
+    ins, outs, reqs = [], [], []
+    for j, bits in enumerate(bits_cases):
+        k = torch.empty(n, dtype=torch.uint32, device=DEV)
+        if bits == 0:
+            raja.fill(raja.FILL_U32_CONST, k, a=12345)
+        else:
+            raja.fill(raja.FILL_U32_BITS, k, seed=90 + j, a=bits)
+        v = torch.arange(n, dtype=torch.int32, device=DEV)
+        ok = torch.full((n,), 0xFFFFFFFF, dtype=torch.int64, device=DEV).to(torch.uint32)
+        ov = torch.full((n,), -1, dtype=torch.int32, device=DEV)
+        ins.append((k, v))
+        outs.append((ok, ov))
+        reqs.append(raja.request(raja.REQ_SORT_PAIRS_U32_I32, n, k, ok, v, ov))
+    print("serve sorts with constant digits: %d requests of %d pairs" % (len(reqs), n), flush=True)
+    srv = raja.Server(raja.SERVE_PER_REQUEST, streams=1, max_inflight=4, max_batch=1,
+                      batch_limit=0, arena_bytes=1 << 30)
+    res, _ = srv.run(reqs)
+    srv.close()
+    assert all(r.status == 0 and r.unit_size == 1 for r in res)
+    for bits, (k, v), (ok, ov) in zip(bits_cases, ins, outs):
+        hk = k.view(torch.int32).cpu().numpy().view(np.uint32)
+        ek, ev = orc.seq_stable_sort_pairs(hk, np.arange(n, dtype=np.int32))
+        assert np.array_equal(ok.view(torch.int32).cpu().numpy().view(np.uint32), ek), bits
+        assert np.array_equal(ov.cpu().numpy(), ev), bits
+        # inputs untouched
+        assert np.array_equal(v.cpu().numpy(), np.arange(n, dtype=np.int32)), bits
