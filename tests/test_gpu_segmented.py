@@ -245,3 +245,33 @@ def test_segmented_sort_long_segments_one_pass_per_digit(raja, kdt, with_vals, d
         if with_vals:
             assert np.array_equal(gv[a:b], (a + idx).astype(np.int32)), s
     plan.close()
+
+
+def test_segmented_sort_many_tiles_matches_span_sorts(raja):
+    """A segmented onesweep over more than 8192 tiles (about 1.4e8 pairs in
+    600 segments), where each histogram block takes 4 consecutive tile
+    records and adds to several segments' rows: every segment must equal the
+    span sort of its own pairs, bit for bit."""
+    rng = np.random.default_rng(77)
+    lens = rng.integers(120000, 350000, 600)
+    lens[:5] = [0, 1, 8193, 16384, 16385]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(off[-1])
+    tile = 16384
+    assert int(sum((int(x) + tile - 1) // tile for x in lens)) >= 4 * 2048
+    print("segmented onesweep, many tiles: %d pairs in %d segments" % (n, len(lens)), flush=True)
+    k = torch.empty(n, dtype=torch.uint32, device=DEV)
+    raja.fill(raja.FILL_U32_BITS, k, seed=78, a=32)
+    v = torch.arange(n, dtype=torch.int32, device=DEV)
+    k0, v0 = k.clone(), v.clone()
+    plan = raja.Segments(off)
+    raja.segmented_sort_pairs(plan, k, v)
+    torch.cuda.synchronize()
+    for s in range(len(lens)):
+        a, b = int(off[s]), int(off[s + 1])
+        kk, vv = k0[a:b].clone(), v0[a:b].clone()
+        if b - a > 1:
+            raja.sort_pairs(kk, vv)
+        assert torch.equal(k[a:b].view(torch.int32), kk.view(torch.int32)), s
+        assert torch.equal(v[a:b], vv), s
+    plan.close()
